@@ -1,0 +1,183 @@
+/*
+ * lj_engine.h -- C ABI of the MI355X (gfx950) NVE Lennard-Jones step engine.
+ *
+ * The engine replaces, for one or more independent periodic systems
+ * ("replicas") resident on one GPU, the reference's per-step operator chain
+ *
+ *   engine::VelocityVerletIntegrator::operator()   engine/integrator.cpp:62-88
+ *   engine::PeriodicBoundaryCondition::operator()  engine/boundary_condition.cpp:37-51
+ *   engine::ShortRangeForceCalculation::operator() engine/force_calculation.cpp:46-71
+ *   engine::CellListParticlePairFilter::pairs      engine/particle_pair_filter.cpp:131-192
+ *   physics::LennardJonesForce::compute            physics/lennard_jones_force.cpp:76-128
+ *   physics::ThermodynamicMeasurement::operator()  physics/measurements.hpp:59-70
+ *   physics::set_temperature                       physics/transformations.cpp:94-117
+ *
+ * (paths relative to the reference repository root).  Plain C types only;
+ * no exception crosses this boundary.  Every function returns LJ_OK (0) or a
+ * negative LJ_E* code; lj_last_error() describes the most recent failure on
+ * the calling thread.  One engine handle owns its device buffers and one HIP
+ * stream; use a handle from one host thread at a time.
+ *
+ * Host arrays are row-major N x 3 doubles (x, y, z per particle), indexed by
+ * particle id -- the columns of the reference's 4 x N Eigen matrices
+ * (physics/system_state.hpp:85-90) without the always-zero 4th row.
+ */
+#ifndef LJ_ENGINE_H
+#define LJ_ENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LJ_OK 0
+#define LJ_E_INVALID -1   /* bad argument or configuration */
+#define LJ_E_HIP -2       /* HIP runtime failure */
+#define LJ_E_STATE -3     /* call not valid in the engine's current state */
+
+/* Arithmetic of the pair kernel.  LJ_FP64: everything in double.  LJ_MIXED:
+ * pair separations, r^-6 and forces in float on cell-local coordinates;
+ * positions, velocities, displacements and the PE / virial / KE / MSD sums
+ * stay in double. */
+#define LJ_PRECISION_FP64 0
+#define LJ_PRECISION_MIXED 1
+
+/* Pair law.  LJ_LAW_LJ is the reference's spline-shifted LJ
+ * (physics/lennard_jones_force.cpp:33-128) generalised to V = eps v(r/sigma)
+ * with the cutoff rc = cutoff * sigma; eps = sigma = 1 is the reference.
+ * LJ_LAW_CONSTANT is the reference's test double
+ * tests/cpp/mock/constant_short_range_force.cpp:31-59 (V = k (rc - r),
+ * W = k r).  LJ_LAW_NONE performs no force calculation: forces keep their
+ * stored values (the reference's Integrator built without a short-range
+ * force, engine/integrator.cpp:79). */
+#define LJ_LAW_LJ 0
+#define LJ_LAW_CONSTANT 1
+#define LJ_LAW_NONE 2
+
+typedef struct lj_system_desc {
+    int particle_count;       /* N >= 1 */
+    double box[3];            /* periodic box sides (tools/bounding_box.hpp:30-61) */
+    double cutoff;            /* LJ: reduced cutoff rc/sigma; CONSTANT: absolute cutoff */
+    double time_delta;        /* dt */
+    double epsilon;           /* LJ energy scale (1 in the reference) */
+    double sigma;             /* LJ length scale (1 in the reference) */
+    double strength;          /* CONSTANT law strength k */
+    int force_law;            /* LJ_LAW_* */
+    int periodic;             /* 1: apply the periodic boundary condition each step;
+                                 0 only allowed with LJ_LAW_NONE */
+} lj_system_desc;
+
+/* One row of the reference's thermodynamics log
+ * (physics/measurements.hpp:42-56): the state after a step. */
+typedef struct lj_thermo {
+    double time;
+    double kinetic_energy;
+    double potential_energy;
+    double total_energy;
+    double virial;
+    double temperature;
+    double mean_square_displacement;
+} lj_thermo;
+
+typedef struct lj_engine lj_engine;
+
+/* Message of the most recent failing call on this thread ("" if none). */
+const char* lj_last_error(void);
+
+/* Number of visible HIP devices (0 if none); never fails. */
+int lj_device_count(void);
+
+/* Creates an engine for nsys independent systems on the current HIP
+ * device.  Replaces api::Simulation's construction of the integrator
+ * (api/simulation.cpp:119-125, Integrator::Builder(dt).bounding_box(box)
+ * .short_range_force(lj).build(), engine/integrator_builder.hpp:36-170) and,
+ * for nsys > 1, the per-replica integrators of api::SimulationPool
+ * (api/simulation_pool.cpp:93-101).  The reference's preconditions are
+ * errors here: box >= cutoff on every axis (engine/particle_pair_filter.cpp:61-64),
+ * LJ cutoff^2 > 2^(1/3) (physics/lennard_jones_force.cpp:45). */
+int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_engine** out);
+
+int lj_engine_destroy(lj_engine* e);
+
+int lj_engine_system_count(const lj_engine* e);
+
+/* Loads the state of system `sys` (the reference's SystemState,
+ * physics/system_state.hpp:35-102): pos, vel, disp, force as N x 3 host
+ * arrays (disp or force may be NULL = zeros) plus the elapsed time. */
+int lj_engine_set_state(lj_engine* e, int sys, const double* pos, const double* vel,
+                        const double* disp, const double* force, double time);
+
+/* Copies the state of system `sys` back, in particle-id order.  Any output
+ * pointer may be NULL.  potential_energy / virial are those of the last
+ * force calculation. */
+int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double* disp,
+                        double* force, double* time, double* potential_energy, double* virial);
+
+/* Advances every system by nsteps velocity-Verlet steps
+ * (engine/integrator.cpp:51-88 applied nsteps times).  If records is not
+ * NULL it receives nsteps * nsys lj_thermo rows, step-major
+ * (records[step * nsys + sys]), each the measurement of
+ * physics/measurements.hpp:59-70 after that step.  The loop runs from a
+ * captured HIP graph unless graphs were disabled. */
+int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records);
+
+/* Recomputes forces, potential energy and virial on the current positions
+ * without moving the particles (engine/force_calculation.cpp:46-71).
+ * pe / vir receive nsys values each when not NULL. */
+int lj_engine_compute_forces(lj_engine* e, double* pe, double* vir);
+
+/* physics::set_temperature (physics/transformations.cpp:94-117) on system
+ * `sys`: v *= sqrt(target / T_instantaneous).  measured (optional) receives
+ * the temperature before scaling. */
+int lj_engine_set_temperature(lj_engine* e, int sys, double target, double* measured);
+
+/* ThermodynamicMeasurement of the current state (physics/measurements.hpp:59-70)
+ * for every system, without stepping; out receives nsys rows. */
+int lj_engine_measure(lj_engine* e, lj_thermo* out);
+
+/* ---- parity hooks ---------------------------------------------------- */
+
+/* Integer cell coordinates of every particle of system `sys` on the current
+ * positions, as engine/particle_pair_filter.cpp:144-147 computes them
+ * (keys: N x 3 ints, particle-id order).  nc (optional) receives the grid
+ * shape of tools/cell_list_array.cpp:45-51.  Returns LJ_OK; *clamped
+ * (optional) receives the number of coordinates clamped into [0, nc). */
+int lj_engine_cell_keys(lj_engine* e, int sys, int* keys, int* nc, int* clamped);
+
+/* Cell list of system `sys` built by the engine's counting sort on the
+ * current positions, in CSR form: start[ncell + 1] (offsets from 0),
+ * atoms[N] (particle ids, ascending within each cell; cells x-slowest as in
+ * tools/cell_list_array.cpp:63-81). */
+int lj_engine_cell_list(lj_engine* e, int sys, int* start, int* atoms);
+
+/* In-cutoff pairs of system `sys` found by the engine's pair traversal on
+ * the current positions.  Each unordered pair appears once as
+ * (first < second, image x, image y, image z) in idx[5*k..] with separation
+ * r_first - r_second - image * L in sep[3*k..] (the reference's ParticlePair,
+ * engine/particle_pair_filter.hpp:35-50, oriented by id).  *count receives
+ * the number of pairs; at most cap are written (order unspecified). */
+int lj_engine_pairs(lj_engine* e, int sys, long long cap, int* idx, double* sep,
+                    long long* count);
+
+/* Ordered-pair distance histogram of every system on the current positions
+ * (bins of width rmax / bins over [0, rmax), images included); accumulates
+ * into the engine's g(r) buffer.  lj_engine_rdf_read copies the counts
+ * (nsys x bins doubles) and the number of accumulated samples, then clears. */
+int lj_engine_rdf_sample(lj_engine* e, double rmax, int bins);
+int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples);
+
+/* ---- tuning ----------------------------------------------------------- */
+
+/* 0 disables HIP graph capture of the step loop (launches each kernel). */
+int lj_engine_set_graphs(lj_engine* e, int enabled);
+
+/* Mean duration in milliseconds of each step-loop kernel over the last
+ * lj_engine_step call with profiling enabled (HIP events on the engine's
+ * stream).  names/ms receive up to cap entries; returns the entry count. */
+int lj_engine_set_profiling(lj_engine* e, int enabled);
+int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LJ_ENGINE_H */

@@ -1,0 +1,869 @@
+// lj_engine.hip -- host side of the C ABI declared in include/lj_engine.h.
+//
+// One engine = device buffers + one HIP stream for one or more independent
+// periodic systems.  A step is the kernel chain
+//   integrate+key -> scan tiles -> scan apply -> scatter -> sort/reorder
+//   -> pair forces + second kick + sums -> finalize
+// (see lj_kernels.hpp); the chain for two steps (the sort alternates between
+// two buffer sets) is captured once into a HIP graph and replayed.
+
+#include "lj_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lj_kernels.hpp"
+
+using ljk::kBlock;
+using ljk::kScanTile;
+using ljk::SysDev;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(LJ_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+template <class T>
+int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void**)p, count * sizeof(T)));
+    return LJ_OK;
+}
+
+struct KernelTimer {
+    const char* name;
+    double total_ms = 0;
+    long long calls = 0;
+};
+
+}  // namespace
+
+struct lj_engine {
+    int nsys = 0;
+    int precision = 0;
+    bool has_cells = true;      // false when every system uses LJ_LAW_NONE
+    std::vector<SysDev> sys;
+    int nslot = 0, nblk = 0;
+    int ncell_total = 0, ntile = 0;
+
+    SysDev* d_sys = nullptr;
+    int* d_blk_sys = nullptr;
+    double* d_pos[2] = {nullptr, nullptr};
+    double* d_vel[2] = {nullptr, nullptr};
+    double* d_disp[2] = {nullptr, nullptr};
+    int* d_id[2] = {nullptr, nullptr};
+    double* d_force = nullptr;
+    int *d_cell_of = nullptr, *d_slot_of = nullptr, *d_perm = nullptr, *d_scell = nullptr;
+    float4* d_lpos = nullptr;
+    int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
+    double* d_partial = nullptr;
+    double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
+    int* d_step = nullptr;
+    double* d_records = nullptr;
+    int rec_cap = 0;
+    double* d_measure = nullptr;
+    unsigned long long* d_rdf = nullptr;
+    int rdf_bins = 0, rdf_samples = 0;
+    double rdf_rmax = 0;
+
+    int cur = 0;                // which buffer set holds the current state
+    bool sorted = false;        // slots are in cell order (after a cell build)
+    hipStream_t stream = nullptr;
+    bool use_graphs = true;
+    hipGraphExec_t graph2 = nullptr;   // two steps starting from buffer set 0
+    int graph_cap = -1;               // record capacity baked into the graph
+
+    bool profiling = false;
+    std::vector<KernelTimer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ljk::StepArgs args(int from) const
+    {
+        ljk::StepArgs p;
+        p.sys = d_sys;
+        p.blk_sys = d_blk_sys;
+        auto planes = [&](double* base) {
+            return ljk::Planes{base, base + nslot, base + 2 * (size_t)nslot};
+        };
+        p.pos = planes(d_pos[from]);
+        p.vel = planes(d_vel[from]);
+        p.disp = planes(d_disp[from]);
+        p.force = planes(d_force);
+        p.pos_n = planes(d_pos[from ^ 1]);
+        p.vel_n = planes(d_vel[from ^ 1]);
+        p.disp_n = planes(d_disp[from ^ 1]);
+        p.id = d_id[from];
+        p.id_n = d_id[from ^ 1];
+        p.cell_of = d_cell_of;
+        p.slot_of = d_slot_of;
+        p.count = d_count;
+        p.count_init = d_count_init;
+        p.start = d_start;
+        p.tile_sum = d_tile;
+        p.perm = d_perm;
+        p.scell = d_scell;
+        p.lpos = d_lpos;
+        p.partial = d_partial;
+        p.ncell_total = ncell_total;
+        p.nslot = nslot;
+        return p;
+    }
+
+    ljk::FinalArgs fargs(int mode) const
+    {
+        ljk::FinalArgs f;
+        f.sys = d_sys;
+        f.partial = d_partial;
+        f.time = d_time;
+        f.pe = d_pe;
+        f.vir = d_vir;
+        f.step = d_step;
+        f.records = d_records;
+        f.measure = d_measure;
+        f.nsys = nsys;
+        f.cap = rec_cap;
+        f.mode = mode;
+        return f;
+    }
+};
+
+namespace {
+
+// Launch helper: optional per-kernel HIP-event timing (eager launches only).
+template <class F>
+int timed(lj_engine* e, int slot, const char* name, F&& launch)
+{
+    if (!e->profiling) {
+        launch();
+        return LJ_OK;
+    }
+    if ((int)e->timers.size() <= slot) e->timers.resize(slot + 1);
+    e->timers[slot].name = name;
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    launch();
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->timers[slot].total_ms += ms;
+    e->timers[slot].calls += 1;
+    return LJ_OK;
+}
+
+// Cell build on the current positions: optional integration (first half
+// kick, drift, wrap) fused with the keys, then scan, scatter and the sorted
+// move into the other buffer set.  Leaves e->cur pointing at the sorted set.
+int enqueue_cells(lj_engine* e, bool integrate)
+{
+    const ljk::StepArgs p = e->args(e->cur);
+    const dim3 gb(e->nblk), tb(kBlock);
+    int rc = LJ_OK;
+    if (integrate)
+        rc = timed(e, 0, "integrate_key", [&] {
+            hipLaunchKernelGGL((ljk::k_integrate_key<true, true>), gb, tb, 0, e->stream, p);
+        });
+    else
+        rc = timed(e, 0, "integrate_key", [&] {
+            hipLaunchKernelGGL((ljk::k_integrate_key<false, true>), gb, tb, 0, e->stream, p);
+        });
+    if (rc) return rc;
+    if ((rc = timed(e, 1, "scan_tiles", [&] {
+             hipLaunchKernelGGL(ljk::k_scan_tiles, dim3(e->ntile), tb, 0, e->stream, p);
+         })))
+        return rc;
+    if ((rc = timed(e, 2, "scan_apply", [&] {
+             hipLaunchKernelGGL(ljk::k_scan_apply, dim3(e->ntile), tb, 0, e->stream, p);
+         })))
+        return rc;
+    if ((rc = timed(e, 3, "scatter", [&] {
+             hipLaunchKernelGGL(ljk::k_scatter, gb, tb, 0, e->stream, p);
+         })))
+        return rc;
+    if (e->precision == LJ_PRECISION_MIXED)
+        rc = timed(e, 4, "sort_reorder", [&] {
+            hipLaunchKernelGGL((ljk::k_sort_reorder<true>), gb, tb, 0, e->stream, p);
+        });
+    else
+        rc = timed(e, 4, "sort_reorder", [&] {
+            hipLaunchKernelGGL((ljk::k_sort_reorder<false>), gb, tb, 0, e->stream, p);
+        });
+    if (rc) return rc;
+    e->cur ^= 1;
+    e->sorted = true;
+    return LJ_OK;
+}
+
+template <int PREC, int LAW, bool KICK>
+void launch_pair(lj_engine* e, const ljk::StepArgs& p)
+{
+    hipLaunchKernelGGL((ljk::k_pair<PREC, LAW, KICK>), dim3(e->nblk), dim3(kBlock), 0, e->stream,
+                       p);
+}
+
+// Pair forces (or, for LJ_LAW_NONE engines / measure, just the sums).
+int enqueue_pair(lj_engine* e, bool kick, bool measure_only)
+{
+    const ljk::StepArgs p = e->args(e->cur);
+    const int law = measure_only ? ljk::LAW_NONE : e->sys[0].law;
+    const bool mixed = e->precision == LJ_PRECISION_MIXED;
+    return timed(e, 5, "pair", [&] {
+        if (law == ljk::LAW_NONE) {
+            if (kick) launch_pair<ljk::FP64, ljk::LAW_NONE, true>(e, p);
+            else launch_pair<ljk::FP64, ljk::LAW_NONE, false>(e, p);
+        } else if (law == ljk::LAW_LJ) {
+            if (mixed) {
+                if (kick) launch_pair<ljk::MIXED, ljk::LAW_LJ, true>(e, p);
+                else launch_pair<ljk::MIXED, ljk::LAW_LJ, false>(e, p);
+            } else {
+                if (kick) launch_pair<ljk::FP64, ljk::LAW_LJ, true>(e, p);
+                else launch_pair<ljk::FP64, ljk::LAW_LJ, false>(e, p);
+            }
+        } else {
+            if (mixed) {
+                if (kick) launch_pair<ljk::MIXED, ljk::LAW_CONSTANT, true>(e, p);
+                else launch_pair<ljk::MIXED, ljk::LAW_CONSTANT, false>(e, p);
+            } else {
+                if (kick) launch_pair<ljk::FP64, ljk::LAW_CONSTANT, true>(e, p);
+                else launch_pair<ljk::FP64, ljk::LAW_CONSTANT, false>(e, p);
+            }
+        }
+    });
+}
+
+int enqueue_finalize(lj_engine* e, int mode)
+{
+    const ljk::FinalArgs f = e->fargs(mode);
+    return timed(e, 6, "finalize", [&] {
+        hipLaunchKernelGGL(ljk::k_finalize, dim3(e->nsys), dim3(kBlock), 0, e->stream, f);
+    });
+}
+
+// One velocity-Verlet step.
+int enqueue_step(lj_engine* e)
+{
+    int rc;
+    if (e->has_cells) {
+        if ((rc = enqueue_cells(e, true))) return rc;
+    } else {
+        const ljk::StepArgs p = e->args(e->cur);
+        if ((rc = timed(e, 0, "integrate_key", [&] {
+                 hipLaunchKernelGGL((ljk::k_integrate_key<true, false>), dim3(e->nblk),
+                                    dim3(kBlock), 0, e->stream, p);
+             })))
+            return rc;
+    }
+    if ((rc = enqueue_pair(e, true, false))) return rc;
+    return enqueue_finalize(e, 0);
+}
+
+int ensure_records(lj_engine* e, int nsteps)
+{
+    if (nsteps <= e->rec_cap) return LJ_OK;
+    int cap = std::max(nsteps, 64);
+    if (e->d_records) HIP_TRY(hipFree(e->d_records));
+    e->d_records = nullptr;
+    int rc = dev_alloc(&e->d_records, (size_t)cap * e->nsys * 7);
+    if (rc) return rc;
+    e->rec_cap = cap;
+    return LJ_OK;
+}
+
+int build_graph(lj_engine* e)
+{
+    if (e->graph2 && e->graph_cap == e->rec_cap) return LJ_OK;
+    if (e->graph2) {
+        HIP_TRY(hipGraphExecDestroy(e->graph2));
+        e->graph2 = nullptr;
+    }
+    const int saved = e->cur;
+    const bool saved_sorted = e->sorted;
+    e->cur = 0;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(e);
+    if (!rc) rc = enqueue_step(e);
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    e->cur = saved;
+    e->sorted = saved_sorted;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return fail(LJ_E_HIP, std::string("hipStreamEndCapture: ") +
+                                                    hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&e->graph2, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(LJ_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    e->graph_cap = e->rec_cap;
+    return LJ_OK;
+}
+
+int check_engine(const lj_engine* e) { return e ? LJ_OK : fail(LJ_E_INVALID, "null engine"); }
+
+int check_sys(const lj_engine* e, int sys)
+{
+    if (!e) return fail(LJ_E_INVALID, "null engine");
+    if (sys < 0 || sys >= e->nsys) return fail(LJ_E_INVALID, "system index out of range");
+    return LJ_OK;
+}
+
+// Ensures slots are in cell order without moving particles (needed before
+// any traversal when the state was just loaded).
+int ensure_sorted(lj_engine* e)
+{
+    if (!e->has_cells || e->sorted) return LJ_OK;
+    return enqueue_cells(e, false);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lj_last_error(void) { return g_err.c_str(); }
+
+int lj_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_engine** out)
+{
+    if (!out) return fail(LJ_E_INVALID, "out is null");
+    *out = nullptr;
+    if (!systems || nsys < 1) return fail(LJ_E_INVALID, "need at least one system");
+    if (precision != LJ_PRECISION_FP64 && precision != LJ_PRECISION_MIXED)
+        return fail(LJ_E_INVALID, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(LJ_E_HIP, "no HIP device visible");
+
+    auto e = new lj_engine();
+    e->nsys = nsys;
+    e->precision = precision;
+    int nnone = 0;
+    for (int k = 0; k < nsys; ++k) nnone += systems[k].force_law == LJ_LAW_NONE;
+    if (nnone != 0 && nnone != nsys) {
+        delete e;
+        return fail(LJ_E_INVALID, "LJ_LAW_NONE systems cannot share an engine with others");
+    }
+    e->has_cells = nnone == 0;
+    for (int k = 1; k < nsys; ++k)
+        if (systems[k].force_law != systems[0].force_law) {
+            delete e;
+            return fail(LJ_E_INVALID, "all systems of an engine must use the same force law");
+        }
+
+    int atom_off = 0, cell_off = 0, blk = 0;
+    std::vector<int> count_init;
+    for (int k = 0; k < nsys; ++k) {
+        const lj_system_desc& d = systems[k];
+        SysDev s;
+        std::memset(&s, 0, sizeof(s));
+        if (d.particle_count < 1) { delete e; return fail(LJ_E_INVALID, "particle_count < 1"); }
+        if (!(d.time_delta > 0)) { delete e; return fail(LJ_E_INVALID, "time_delta must be > 0"); }
+        if (d.force_law < 0 || d.force_law > 2) { delete e; return fail(LJ_E_INVALID, "bad force_law"); }
+        if (!d.periodic && d.force_law != LJ_LAW_NONE) {
+            delete e;
+            return fail(LJ_E_INVALID, "a force calculation needs the periodic box");
+        }
+        s.n = d.particle_count;
+        s.law = d.force_law;
+        s.periodic = d.periodic ? 1 : 0;
+        s.dt = d.time_delta;
+        s.eps = d.epsilon;
+        s.sigma2 = d.sigma * d.sigma;
+        s.strength = d.strength;
+        for (int a = 0; a < 3; ++a) s.L[a] = d.box[a];
+        double rc_abs = 0;
+        if (d.force_law == LJ_LAW_LJ) {
+            if (!(d.sigma > 0) || !(d.epsilon > 0)) {
+                delete e;
+                return fail(LJ_E_INVALID, "epsilon and sigma must be > 0");
+            }
+            if (!(d.cutoff * d.cutoff > std::cbrt(2.0))) {
+                delete e;
+                return fail(LJ_E_INVALID, "Cutoff distance is too short");
+            }
+            // physics/lennard_jones_force.cpp:68-73 on the reduced cutoff
+            s.rc2_red = d.cutoff * d.cutoff;
+            double rcm6 = 1 / (s.rc2_red * s.rc2_red * s.rc2_red);
+            s.alpha = -4.0 * rcm6 * (rcm6 - 1.0);
+            s.beta = 12.0 * rcm6 * (2.0 * rcm6 - 1.0);
+            rc_abs = d.cutoff * d.sigma;
+        } else if (d.force_law == LJ_LAW_CONSTANT) {
+            rc_abs = d.cutoff;
+            s.rc = d.cutoff;
+        }
+        s.rc2 = rc_abs * rc_abs;
+        if (d.force_law != LJ_LAW_NONE) {
+            for (int a = 0; a < 3; ++a) {
+                if (!(d.box[a] >= rc_abs) || !(rc_abs > 0)) {
+                    delete e;
+                    return fail(LJ_E_INVALID, "Simulation box size is less than the cutoff distance");
+                }
+                s.nc[a] = (int)std::floor(d.box[a] / rc_abs);   // tools/cell_list_array.cpp:48
+                s.q[a] = (double)s.nc[a] / d.box[a];
+                s.a[a] = d.box[a] / (double)s.nc[a];
+            }
+            s.ncell = s.nc[0] * s.nc[1] * s.nc[2];
+        }
+        const int slots = round_up(s.n, kBlock);
+        s.atom_off = atom_off;
+        s.blk_first = blk;
+        s.nblk = slots / kBlock;
+        if (e->has_cells) {
+            s.cell_off = cell_off;
+            count_init.resize(cell_off + s.ncell + 1, 0);
+            count_init[cell_off + s.ncell] = slots - s.n;   // tail cell: padding slots
+            cell_off += s.ncell + 1;
+        }
+        atom_off += slots;
+        blk += s.nblk;
+        e->sys.push_back(s);
+    }
+    e->nslot = atom_off;
+    e->nblk = blk;
+    e->ncell_total = cell_off;
+    e->ntile = std::max(1, (cell_off + kScanTile - 1) / kScanTile);
+
+    std::vector<int> blk_sys(blk);
+    for (int k = 0; k < nsys; ++k)
+        for (int b = 0; b < e->sys[k].nblk; ++b) blk_sys[e->sys[k].blk_first + b] = k;
+
+    auto cleanup_fail = [&](int rc) {
+        lj_engine_destroy(e);
+        return rc;
+    };
+    int rc;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup_fail(fail(LJ_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return cleanup_fail(fail(LJ_E_HIP, "hipEventCreate failed"));
+    const size_t ns = e->nslot;
+    if ((rc = dev_alloc(&e->d_sys, nsys)) || (rc = dev_alloc(&e->d_blk_sys, blk)) ||
+        (rc = dev_alloc(&e->d_force, 3 * ns)) || (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
+        (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
+        (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
+        (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
+        return cleanup_fail(rc);
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dev_alloc(&e->d_pos[b], 3 * ns)) || (rc = dev_alloc(&e->d_vel[b], 3 * ns)) ||
+            (rc = dev_alloc(&e->d_disp[b], 3 * ns)) || (rc = dev_alloc(&e->d_id[b], ns)))
+            return cleanup_fail(rc);
+    }
+    if (e->has_cells) {
+        if ((rc = dev_alloc(&e->d_cell_of, ns)) || (rc = dev_alloc(&e->d_slot_of, ns)) ||
+            (rc = dev_alloc(&e->d_perm, ns)) || (rc = dev_alloc(&e->d_scell, ns)) ||
+            (rc = dev_alloc(&e->d_count, e->ncell_total)) ||
+            (rc = dev_alloc(&e->d_count_init, e->ncell_total)) ||
+            (rc = dev_alloc(&e->d_start, e->ncell_total + 1)) ||
+            (rc = dev_alloc(&e->d_tile, e->ntile)))
+            return cleanup_fail(rc);
+        if (precision == LJ_PRECISION_MIXED && (rc = dev_alloc(&e->d_lpos, ns)))
+            return cleanup_fail(rc);
+    }
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    std::vector<double> zeros(std::max<size_t>(3 * ns, (size_t)nsys), 0.0);
+    // particle ids are local to their system: slot atom_off + i holds id i
+    std::vector<int> ident(ns, 0);
+    for (int k = 0; k < nsys; ++k)
+        for (int i = 0; i < e->sys[k].nblk * kBlock; ++i)
+            ident[e->sys[k].atom_off + i] = i < e->sys[k].n ? i : 0;
+    bool ok = h2d(e->d_sys, e->sys.data(), nsys * sizeof(SysDev)) &&
+              h2d(e->d_blk_sys, blk_sys.data(), blk * sizeof(int)) &&
+              h2d(e->d_force, zeros.data(), 3 * ns * sizeof(double)) &&
+              h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
+              h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
+              h2d(e->d_vir, zeros.data(), nsys * sizeof(double));
+    for (int b = 0; b < 2 && ok; ++b)
+        ok = h2d(e->d_pos[b], zeros.data(), 3 * ns * sizeof(double)) &&
+             h2d(e->d_vel[b], zeros.data(), 3 * ns * sizeof(double)) &&
+             h2d(e->d_disp[b], zeros.data(), 3 * ns * sizeof(double)) &&
+             h2d(e->d_id[b], ident.data(), ns * sizeof(int));
+    if (ok && e->has_cells) {
+        ok = h2d(e->d_count, count_init.data(), e->ncell_total * sizeof(int)) &&
+             h2d(e->d_count_init, count_init.data(), e->ncell_total * sizeof(int));
+    }
+    if (!ok) return cleanup_fail(fail(LJ_E_HIP, "initial upload failed"));
+    if ((rc = ensure_records(e, 64))) return cleanup_fail(rc);
+    e->sorted = false;
+    *out = e;
+    return LJ_OK;
+}
+
+int lj_engine_destroy(lj_engine* e)
+{
+    if (!e) return LJ_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
+    void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
+                    e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_start, e->d_tile,
+                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_records,
+                    e->d_measure, e->d_rdf};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (int b = 0; b < 2; ++b) {
+        void* more[] = {e->d_pos[b], e->d_vel[b], e->d_disp[b], e->d_id[b]};
+        for (void* m : more)
+            if (m) (void)hipFree(m);
+    }
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return LJ_OK;
+}
+
+int lj_engine_system_count(const lj_engine* e) { return e ? e->nsys : 0; }
+
+int lj_engine_set_state(lj_engine* e, int sys, const double* pos, const double* vel,
+                        const double* disp, const double* force, double time)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    if (!pos || !vel) return fail(LJ_E_INVALID, "pos and vel are required");
+    const SysDev& S = e->sys[sys];
+    const int n = S.n;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    // Gather the current slot -> id map so other systems stay untouched; the
+    // loaded system goes into its slots in id order (unsorted).
+    std::vector<double> planes(3 * (size_t)n);
+    auto put = [&](double* base, const double* src) -> int {
+        for (int a = 0; a < 3; ++a)
+            for (int i = 0; i < n; ++i) planes[(size_t)a * n + i] = src ? src[3 * (size_t)i + a] : 0.0;
+        for (int a = 0; a < 3; ++a)
+            HIP_TRY(hipMemcpy(base + (size_t)a * e->nslot + S.atom_off, planes.data() + (size_t)a * n,
+                              n * sizeof(double), hipMemcpyHostToDevice));
+        return LJ_OK;
+    };
+    if ((rc = put(e->d_pos[e->cur], pos)) || (rc = put(e->d_vel[e->cur], vel)) ||
+        (rc = put(e->d_disp[e->cur], disp)) || (rc = put(e->d_force, force)))
+        return rc;
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) ids[i] = i;
+    HIP_TRY(hipMemcpy(e->d_id[e->cur] + S.atom_off, ids.data(), n * sizeof(int),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_time + sys, &time, sizeof(double), hipMemcpyHostToDevice));
+    double zero = 0.0;
+    HIP_TRY(hipMemcpy(e->d_pe + sys, &zero, sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_vir + sys, &zero, sizeof(double), hipMemcpyHostToDevice));
+    e->sorted = false;
+    return LJ_OK;
+}
+
+int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double* disp,
+                        double* force, double* time, double* potential_energy, double* virial)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    const SysDev& S = e->sys[sys];
+    const int n = S.n;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<int> ids(n);
+    HIP_TRY(hipMemcpy(ids.data(), e->d_id[e->cur] + S.atom_off, n * sizeof(int),
+                      hipMemcpyDeviceToHost));
+    std::vector<double> planes(3 * (size_t)n);
+    auto get = [&](const double* base, double* dst) -> int {
+        if (!dst) return LJ_OK;
+        for (int a = 0; a < 3; ++a)
+            HIP_TRY(hipMemcpy(planes.data() + (size_t)a * n, base + (size_t)a * e->nslot + S.atom_off,
+                              n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int s = 0; s < n; ++s)
+            for (int a = 0; a < 3; ++a) dst[3 * (size_t)ids[s] + a] = planes[(size_t)a * n + s];
+        return LJ_OK;
+    };
+    if ((rc = get(e->d_pos[e->cur], pos)) || (rc = get(e->d_vel[e->cur], vel)) ||
+        (rc = get(e->d_disp[e->cur], disp)) || (rc = get(e->d_force, force)))
+        return rc;
+    if (time) HIP_TRY(hipMemcpy(time, e->d_time + sys, sizeof(double), hipMemcpyDeviceToHost));
+    if (potential_energy)
+        HIP_TRY(hipMemcpy(potential_energy, e->d_pe + sys, sizeof(double), hipMemcpyDeviceToHost));
+    if (virial) HIP_TRY(hipMemcpy(virial, e->d_vir + sys, sizeof(double), hipMemcpyDeviceToHost));
+    return LJ_OK;
+}
+
+int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(LJ_E_INVALID, "nsteps < 0");
+    if (nsteps == 0) return LJ_OK;
+    if ((rc = ensure_records(e, nsteps))) return rc;
+    hipLaunchKernelGGL(ljk::k_reset_steps, dim3((e->nsys + 255) / 256), dim3(256), 0, e->stream,
+                       e->d_step, e->nsys);
+    int done = 0;
+    const bool graphs = e->use_graphs && !e->profiling && e->has_cells;
+    if (graphs && nsteps >= 2) {
+        if ((rc = build_graph(e))) return rc;
+        // the graph starts from buffer set 0: take one eager step if needed
+        if (e->cur != 0) {
+            if ((rc = enqueue_step(e))) return rc;
+            ++done;
+        }
+        while (nsteps - done >= 2) {
+            HIP_TRY(hipGraphLaunch(e->graph2, e->stream));
+            done += 2;
+            e->sorted = true;
+        }
+    }
+    while (done < nsteps) {
+        if ((rc = enqueue_step(e))) return rc;
+        ++done;
+    }
+    HIP_TRY(hipGetLastError());
+    if (records)
+        HIP_TRY(hipMemcpyAsync(records, e->d_records, (size_t)nsteps * e->nsys * sizeof(lj_thermo),
+                               hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+
+int lj_engine_compute_forces(lj_engine* e, double* pe, double* vir)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (e->has_cells) {
+        if ((rc = enqueue_cells(e, false))) return rc;
+        if ((rc = enqueue_pair(e, false, false))) return rc;
+        if ((rc = enqueue_finalize(e, 1))) return rc;
+    }
+    HIP_TRY(hipGetLastError());
+    if (pe) HIP_TRY(hipMemcpyAsync(pe, e->d_pe, e->nsys * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (vir) HIP_TRY(hipMemcpyAsync(vir, e->d_vir, e->nsys * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+
+int lj_engine_measure(lj_engine* e, lj_thermo* out)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if ((rc = enqueue_pair(e, false, true))) return rc;
+    if ((rc = enqueue_finalize(e, 2))) return rc;
+    HIP_TRY(hipGetLastError());
+    if (out)
+        HIP_TRY(hipMemcpyAsync(out, e->d_measure, e->nsys * sizeof(lj_thermo), hipMemcpyDeviceToHost,
+                               e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+
+int lj_engine_set_temperature(lj_engine* e, int sys, double target, double* measured)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    std::vector<lj_thermo> m(e->nsys);
+    if ((rc = lj_engine_measure(e, m.data()))) return rc;
+    const double t = m[sys].temperature;
+    if (measured) *measured = t;
+    if (!(t > 0)) return fail(LJ_E_STATE, "Cannot scale temperature of zero-temperature state");
+    const double scale = std::sqrt(target / t);
+    const ljk::StepArgs p = e->args(e->cur);
+    hipLaunchKernelGGL(ljk::k_scale_velocities, dim3(e->sys[sys].nblk), dim3(kBlock), 0, e->stream,
+                       p, sys, scale);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+
+int lj_engine_cell_keys(lj_engine* e, int sys, int* keys, int* nc, int* clamped)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    const SysDev& S = e->sys[sys];
+    if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
+    if (nc)
+        for (int a = 0; a < 3; ++a) nc[a] = S.nc[a];
+    int* d_keys = nullptr;
+    int* d_cl = nullptr;
+    if ((rc = dev_alloc(&d_keys, 3 * (size_t)S.n)) || (rc = dev_alloc(&d_cl, 1))) {
+        if (d_keys) (void)hipFree(d_keys);
+        return rc;
+    }
+    int zero = 0;
+    hipError_t he = hipMemcpy(d_cl, &zero, sizeof(int), hipMemcpyHostToDevice);
+    if (he == hipSuccess) {
+        const ljk::StepArgs p = e->args(e->cur);
+        hipLaunchKernelGGL(ljk::k_cell_keys, dim3(S.nblk), dim3(kBlock), 0, e->stream, p, sys,
+                           d_keys, d_cl);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he == hipSuccess && keys)
+        he = hipMemcpy(keys, d_keys, 3 * (size_t)S.n * sizeof(int), hipMemcpyDeviceToHost);
+    int cl = 0;
+    if (he == hipSuccess) he = hipMemcpy(&cl, d_cl, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d_keys);
+    (void)hipFree(d_cl);
+    if (he != hipSuccess) return fail(LJ_E_HIP, std::string("cell_keys: ") + hipGetErrorString(he));
+    if (clamped) *clamped = cl;
+    return LJ_OK;
+}
+
+int lj_engine_cell_list(lj_engine* e, int sys, int* start, int* atoms)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    const SysDev& S = e->sys[sys];
+    if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
+    if ((rc = enqueue_cells(e, false))) return rc;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<int> st(S.ncell + 1);
+    HIP_TRY(hipMemcpy(st.data(), e->d_start + S.cell_off, (S.ncell + 1) * sizeof(int),
+                      hipMemcpyDeviceToHost));
+    if (start)
+        for (int c = 0; c <= S.ncell; ++c) start[c] = st[c] - S.atom_off;
+    if (atoms)
+        HIP_TRY(hipMemcpy(atoms, e->d_id[e->cur] + S.atom_off, S.n * sizeof(int),
+                          hipMemcpyDeviceToHost));
+    return LJ_OK;
+}
+
+int lj_engine_pairs(lj_engine* e, int sys, long long cap, int* idx, double* sep, long long* count)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    const SysDev& S = e->sys[sys];
+    if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
+    if (cap < 0) return fail(LJ_E_INVALID, "cap < 0");
+    if ((rc = ensure_sorted(e))) return rc;
+    int* d_idx = nullptr;
+    double* d_sep = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    if ((rc = dev_alloc(&d_idx, 5 * (size_t)std::max(cap, 1LL))) ||
+        (rc = dev_alloc(&d_sep, 3 * (size_t)std::max(cap, 1LL))) || (rc = dev_alloc(&d_cnt, 1))) {
+        if (d_idx) (void)hipFree(d_idx);
+        if (d_sep) (void)hipFree(d_sep);
+        return rc;
+    }
+    unsigned long long zero = 0, total = 0;
+    hipError_t he = hipMemcpy(d_cnt, &zero, sizeof(zero), hipMemcpyHostToDevice);
+    if (he == hipSuccess) {
+        const ljk::StepArgs p = e->args(e->cur);
+        hipLaunchKernelGGL(ljk::k_pairs_emit, dim3(S.nblk), dim3(kBlock), 0, e->stream, p, sys, cap,
+                           d_idx, d_sep, d_cnt);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he == hipSuccess) he = hipMemcpy(&total, d_cnt, sizeof(total), hipMemcpyDeviceToHost);
+    const long long w = std::min<long long>((long long)total, cap);
+    if (he == hipSuccess && idx && w > 0)
+        he = hipMemcpy(idx, d_idx, 5 * (size_t)w * sizeof(int), hipMemcpyDeviceToHost);
+    if (he == hipSuccess && sep && w > 0)
+        he = hipMemcpy(sep, d_sep, 3 * (size_t)w * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_sep);
+    (void)hipFree(d_cnt);
+    if (he != hipSuccess) return fail(LJ_E_HIP, std::string("pairs: ") + hipGetErrorString(he));
+    if (count) *count = (long long)total;
+    return LJ_OK;
+}
+
+int lj_engine_rdf_sample(lj_engine* e, double rmax, int bins)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->has_cells) return fail(LJ_E_STATE, "no cell grid");
+    if (!(rmax > 0) || bins < 1 || bins > 4096) return fail(LJ_E_INVALID, "bad rmax / bins");
+    if (e->d_rdf && (bins != e->rdf_bins || rmax != e->rdf_rmax))
+        return fail(LJ_E_STATE, "read the accumulated g(r) before changing rmax / bins");
+    if (!e->d_rdf) {
+        if ((rc = dev_alloc(&e->d_rdf, (size_t)e->nsys * bins))) return rc;
+        HIP_TRY(hipMemset(e->d_rdf, 0, (size_t)e->nsys * bins * sizeof(unsigned long long)));
+        e->rdf_bins = bins;
+        e->rdf_rmax = rmax;
+        e->rdf_samples = 0;
+    }
+    if ((rc = ensure_sorted(e))) return rc;
+    const ljk::StepArgs p = e->args(e->cur);
+    hipLaunchKernelGGL(ljk::k_rdf, dim3(e->nblk), dim3(kBlock), bins * sizeof(unsigned int),
+                       e->stream, p, rmax, bins, e->d_rdf);
+    HIP_TRY(hipGetLastError());
+    e->rdf_samples += 1;
+    return LJ_OK;
+}
+
+int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->d_rdf) {
+        if (samples) *samples = 0;
+        return LJ_OK;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<unsigned long long> h((size_t)e->nsys * e->rdf_bins);
+    HIP_TRY(hipMemcpy(h.data(), e->d_rdf, h.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+    if (counts)
+        for (size_t k = 0; k < h.size(); ++k) counts[k] = (double)h[k];
+    if (samples) *samples = e->rdf_samples;
+    HIP_TRY(hipFree(e->d_rdf));
+    e->d_rdf = nullptr;
+    e->rdf_samples = 0;
+    return LJ_OK;
+}
+
+int lj_engine_set_graphs(lj_engine* e, int enabled)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    e->use_graphs = enabled != 0;
+    return LJ_OK;
+}
+
+int lj_engine_set_profiling(lj_engine* e, int enabled)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    e->profiling = enabled != 0;
+    e->timers.clear();
+    return LJ_OK;
+}
+
+int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms, int* count)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    int k = 0;
+    for (const KernelTimer& t : e->timers) {
+        if (!t.calls) continue;
+        if (k < cap) {
+            if (names) names[k] = t.name;
+            if (ms) ms[k] = t.total_ms / (double)t.calls;
+        }
+        ++k;
+    }
+    if (count) *count = k;
+    return LJ_OK;
+}
+
+}  // extern "C"

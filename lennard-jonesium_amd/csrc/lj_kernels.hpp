@@ -1,0 +1,667 @@
+// lj_kernels.hpp -- device code of the gfx950 NVE Lennard-Jones step engine.
+//
+// Layout in HBM (one engine = one or more independent periodic systems):
+//   * particles live in "slots"; system k owns slots [atom_off, atom_off + N)
+//     of a range padded to a multiple of kBlock, so every kBlock-wide block
+//     of slots belongs to exactly one system (blk_sys[] maps block -> system);
+//   * per-particle state is structure-of-arrays double x / y / z planes
+//     (plane stride = total slot count): positions, velocities, displacements
+//     and forces, plus the particle id held in each slot;
+//   * after every cell build the slots of a system are sorted by cell
+//     (cells x-slowest, z-fastest, as tools/cell_list_array.cpp:63-81 visits
+//     them) and, inside a cell, by ascending particle id -- exactly the
+//     membership and order of the reference's std::vector cell lists
+//     (engine/particle_pair_filter.cpp:150-154);
+//   * cells of all systems are numbered consecutively; each system is
+//     followed by one "tail" cell holding its padding slots, so that one
+//     exclusive scan over all cell counts yields every system's slot range.
+//
+// Arithmetic that decides integers (cell keys, the r^2 < rc^2 test) and the
+// periodic wrap is written exactly as the oracle evaluates it; the whole
+// file is compiled with -ffp-contract=off, and fused multiply-adds appear only
+// where they are written explicitly (mixed-precision force math).
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ljk {
+
+constexpr int kBlock = 256;        // slots per block in particle-space kernels
+constexpr int kScanTile = 1024;    // cells per block in the scan kernels
+
+enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
+enum Prec { FP64 = 0, MIXED = 1 };
+
+// Per-system constants (device resident, read through uniform loads).
+struct SysDev {
+    int atom_off;      // first slot
+    int n;             // particle count
+    int cell_off;      // first cell (tail cell at cell_off + ncell)
+    int ncell;         // real cells nc0 * nc1 * nc2
+    int nc[3];
+    int law;
+    int periodic;
+    int blk_first;     // first particle-space block
+    int nblk;          // blocks of this system
+    int pad_;
+    double L[3];       // box
+    double q[3];       // nc / L (the reference forms this quotient first)
+    double a[3];       // L / nc (cell side, used for cell-local coordinates)
+    double rc2;        // absolute filter cutoff squared
+    double rc2_red;    // LJ reduced cutoff squared
+    double alpha, beta;
+    double eps, sigma2;
+    double strength, rc;   // constant law
+    double dt;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums four per-thread doubles over a kBlock-thread block; thread 0 gets them.
+__device__ __forceinline__ void block_sum4(double v[4], double out[4])
+{
+    __shared__ double red[4][kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) red[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double s = 0;
+            for (int w = 0; w < kBlock / 64; ++w) s += red[k][w];
+            out[k] = s;
+        }
+    }
+}
+
+// Integer cell coordinate along one axis: floor(r * (nc / L)), clamped into
+// [0, nc) (the reference indexes out of range for r == L or a rounded-up
+// quotient; clamping keeps such particles in the boundary cell).
+__device__ __forceinline__ int cell_axis(double r, double q, int nc)
+{
+    int k = (int)floor(r * q);
+    k = k < 0 ? 0 : k;
+    return k >= nc ? nc - 1 : k;
+}
+
+__device__ __forceinline__ void cell_coords(int lc, const int nc[3], int& cx, int& cy, int& cz)
+{
+    cz = lc % nc[2];
+    int t = lc / nc[2];
+    cy = t % nc[1];
+    cx = t / nc[1];
+}
+
+// Neighbour cell along one axis for step o in {-1, 0, 1}: the reference's
+// lattice-image rule (tools/cell_list_array.cpp:142-151).
+__device__ __forceinline__ void nb_axis(int c, int o, int nc, int& nb, int& img)
+{
+    int t = c + o;
+    img = (t < 0) ? -1 : (t >= nc ? 1 : 0);
+    nb = t - img * nc;
+}
+
+// ---------------------------------------------------------------------------
+// kernel parameter blocks
+// ---------------------------------------------------------------------------
+
+struct Planes {       // three coordinate planes of one per-slot vector field
+    double* x;
+    double* y;
+    double* z;
+};
+
+struct StepArgs {
+    const SysDev* sys;
+    const int* blk_sys;
+    Planes pos, vel, disp, force;       // current buffers
+    Planes pos_n, vel_n, disp_n;        // next buffers (sort destination)
+    int* id;                            // current ids
+    int* id_n;                          // next ids
+    int* cell_of;                       // per slot (pre-sort order)
+    int* slot_of;                       // rank of arrival inside the cell
+    int* count;                         // per cell, incl. tail cells
+    const int* count_init;             // tail cells hold their padding count
+    int* start;                         // exclusive scan, ncell_total + 1
+    int* tile_sum;
+    int* perm;                          // arrival-ordered slot lists
+    int* scell;                         // cell of each sorted slot
+    float4* lpos;                       // mixed: cell-local float coordinates
+    double* partial;                    // per block: sum v^2, PE, W, sum d^2
+    int ncell_total;
+    int nslot;
+};
+
+// ---------------------------------------------------------------------------
+// 1. kick + drift + wrap + cell key + cell count
+//    engine/integrator.cpp:69-78 and engine/particle_pair_filter.cpp:141-154
+// ---------------------------------------------------------------------------
+template <bool INTEGRATE, bool KEYS>
+__global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
+{
+    const int b = blockIdx.x;
+    const SysDev& S = p.sys[p.blk_sys[b]];
+    const int s = b * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+    if (INTEGRATE) {
+        const double dt = S.dt;
+        // v += (1/2) F dt   (the stored force: zero before the first calculation)
+        double vx = p.vel.x[s] + ((1. / 2.) * p.force.x[s]) * dt;
+        double vy = p.vel.y[s] + ((1. / 2.) * p.force.y[s]) * dt;
+        double vz = p.vel.z[s] + ((1. / 2.) * p.force.z[s]) * dt;
+        // r += v dt ; d += v dt  (same increment)
+        double ix = vx * dt, iy = vy * dt, iz = vz * dt;
+        x = x + ix; y = y + iy; z = z + iz;
+        p.disp.x[s] = p.disp.x[s] + ix;
+        p.disp.y[s] = p.disp.y[s] + iy;
+        p.disp.z[s] = p.disp.z[s] + iz;
+        if (S.periodic) {   // engine/boundary_condition.cpp:45-48
+            x = x - floor(x / S.L[0]) * S.L[0];
+            y = y - floor(y / S.L[1]) * S.L[1];
+            z = z - floor(z / S.L[2]) * S.L[2];
+        }
+        p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+        p.pos.x[s] = x; p.pos.y[s] = y; p.pos.z[s] = z;
+    }
+    if (KEYS) {
+        int kx = cell_axis(x, S.q[0], S.nc[0]);
+        int ky = cell_axis(y, S.q[1], S.nc[1]);
+        int kz = cell_axis(z, S.q[2], S.nc[2]);
+        int c = S.cell_off + (kx * S.nc[1] + ky) * S.nc[2] + kz;
+        p.cell_of[s] = c;
+        p.slot_of[s] = atomicAdd(&p.count[c], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2-3. exclusive scan of the cell counts (tile sums, then per-tile wave64
+//      scans offset by the preceding tiles); counts are reset afterwards.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan of one int per thread (kBlock threads); *total gets the sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* total)
+{
+    __shared__ int wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < p.ncell_total) v += p.count[base + k];
+    int tot;
+    block_excl_scan(v, &tot);
+    if (threadIdx.x == 0) p.tile_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p)
+{
+    __shared__ int s_off;
+    // offset of this tile: sum of the preceding tile sums
+    int acc = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) acc += p.tile_sum[t];
+    int tot;
+    block_excl_scan(acc, &tot);
+    if (threadIdx.x == 0) s_off = tot;
+    __syncthreads();
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    int c[4], v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[k] = (base + k < p.ncell_total) ? p.count[base + k] : 0;
+        v += c[k];
+    }
+    int tile_tot;
+    int ex = block_excl_scan(v, &tile_tot) + s_off;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < p.ncell_total) {
+            p.start[base + k] = ex;
+            p.count[base + k] = p.count_init[base + k];
+            if (base + k == p.ncell_total - 1) p.start[p.ncell_total] = ex + c[k];
+        }
+        ex += c[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. scatter: slot lists per cell in arrival order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs p)
+{
+    const int b = blockIdx.x;
+    const SysDev& S = p.sys[p.blk_sys[b]];
+    const int s = b * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    p.perm[p.start[p.cell_of[s]] + p.slot_of[s]] = s;
+}
+
+// ---------------------------------------------------------------------------
+// 5. order each cell by particle id and move the particles into their
+//    sorted slots (next buffers); mixed precision also writes cell-local
+//    float coordinates.
+// ---------------------------------------------------------------------------
+template <bool MIXED_>
+__global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
+{
+    const int b = blockIdx.x;
+    const SysDev& S = p.sys[p.blk_sys[b]];
+    const int u = b * kBlock + threadIdx.x;
+    if (u - S.atom_off >= S.n) return;
+    const int old = p.perm[u];
+    const int c = p.cell_of[old];
+    const int me = p.id[old];
+    const int c0 = p.start[c], c1 = p.start[c + 1];
+    int rank = 0;
+    for (int q = c0; q < c1; ++q) rank += (p.id[p.perm[q]] < me) ? 1 : 0;
+    const int dst = c0 + rank;
+    const double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
+    p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
+    p.vel_n.x[dst] = p.vel.x[old]; p.vel_n.y[dst] = p.vel.y[old]; p.vel_n.z[dst] = p.vel.z[old];
+    p.disp_n.x[dst] = p.disp.x[old]; p.disp_n.y[dst] = p.disp.y[old];
+    p.disp_n.z[dst] = p.disp.z[old];
+    p.id_n[dst] = me;
+    p.scell[dst] = c;
+    if (MIXED_) {
+        int cx, cy, cz;
+        cell_coords(c - S.cell_off, S.nc, cx, cy, cz);
+        p.lpos[dst] = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                                  (float)(z - cz * S.a[2]), 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6. pair forces (full shell, one thread per sorted particle), fused with the
+//    second half kick and the per-block sums of |v|^2, PE, W, |d|^2.
+//    physics/lennard_jones_force.cpp:76-128, engine/force_calculation.cpp:46-71,
+//    engine/integrator.cpp:81-82, physics/derived_properties.cpp:37-51.
+// ---------------------------------------------------------------------------
+
+// Visits every in-cutoff neighbour j (with its image) of sorted slot s in
+// double precision, evaluating r^2 exactly as the reference's pair filter:
+// sep = (r_i - r_j) - image * L, r^2 = (x^2 + y^2) + z^2, keep if r^2 < rc^2.
+template <class Visit>
+__device__ __forceinline__ void for_each_neighbour_fp64(const SysDev& S, const StepArgs& p,
+                                                        int s, Visit&& visit)
+{
+    int cx, cy, cz;
+    cell_coords(p.scell[s] - S.cell_off, S.nc, cx, cy, cz);
+    const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
+    const double rc2 = S.rc2;
+    for (int ox = -1; ox <= 1; ++ox) {
+        int nbx, ix;
+        nb_axis(cx, ox, S.nc[0], nbx, ix);
+        const double shx = ix * S.L[0];
+        for (int oy = -1; oy <= 1; ++oy) {
+            int nby, iy;
+            nb_axis(cy, oy, S.nc[1], nby, iy);
+            const double shy = iy * S.L[1];
+            for (int oz = -1; oz <= 1; ++oz) {
+                int nbz, iz;
+                nb_axis(cz, oz, S.nc[2], nbz, iz);
+                const double shz = iz * S.L[2];
+                const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+                const int j0 = p.start[nb], j1 = p.start[nb + 1];
+                for (int j = j0; j < j1; ++j) {
+                    if (j == s) continue;
+                    const double dx = (xi - p.pos.x[j]) - shx;
+                    const double dy = (yi - p.pos.y[j]) - shy;
+                    const double dz = (zi - p.pos.z[j]) - shz;
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 < rc2) visit(j, dx, dy, dz, r2, ix, iy, iz);
+                }
+            }
+        }
+    }
+}
+
+template <int LAW>
+__device__ __forceinline__ void pair_fp64(const SysDev& S, double dx, double dy, double dz,
+                                          double r2, double& fx, double& fy, double& fz,
+                                          double& pe, double& w)
+{
+    double vv, ww;
+    if (LAW == LAW_LJ) {
+        const double r2r = (S.sigma2 == 1.0) ? r2 : r2 / S.sigma2;
+        const double s6 = 1.0 / (r2r * r2r * r2r);
+        vv = 4.0 * s6 * (s6 - 1.0) + S.alpha + S.beta * (r2r / S.rc2_red - 1.0);
+        ww = 24.0 * s6 * (2.0 * s6 - 1.0) - 2.0 * S.beta * (r2r / S.rc2_red);
+        if (S.eps != 1.0) { vv = S.eps * vv; ww = S.eps * ww; }
+        fx += ww * dx / r2; fy += ww * dy / r2; fz += ww * dz / r2;
+    } else {   // constant-magnitude test law
+        const double nrm = sqrt(r2);
+        vv = S.strength * (S.rc - nrm);
+        ww = S.strength * nrm;
+        const double n2 = nrm * nrm;
+        fx += ww * dx / n2; fy += ww * dy / n2; fz += ww * dz / n2;
+    }
+    pe += vv;
+    w += ww;
+}
+
+template <int PREC, int LAW, bool KICK>
+__global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
+{
+    const int b = blockIdx.x;
+    const SysDev& S = p.sys[p.blk_sys[b]];
+    const int s = b * kBlock + threadIdx.x;
+    const bool active = (s - S.atom_off) < S.n;
+    double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
+    if (active) {
+        if (LAW == LAW_NONE) {
+            fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s];
+        } else if (PREC == FP64) {
+            for_each_neighbour_fp64(S, p, s, [&](int, double dx, double dy, double dz, double r2,
+                                                 int, int, int) {
+                pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+            });
+        } else {
+            // cell-local float coordinates; sep = (l_i - l_j) + shift, where
+            // shift = (image * nc - step) * a - image * L for the neighbour cell
+            int cx, cy, cz;
+            cell_coords(p.scell[s] - S.cell_off, S.nc, cx, cy, cz);
+            const float4 li = p.lpos[s];
+            const float rc2f = (float)S.rc2;
+            const float inv_s2 = (float)(1.0 / S.sigma2);
+            const float ur = (float)(1.0 / (S.sigma2 * S.rc2_red));
+            const float al = (float)S.alpha, be = (float)S.beta;
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            for (int ox = -1; ox <= 1; ++ox) {
+                int nbx, ix;
+                nb_axis(cx, ox, S.nc[0], nbx, ix);
+                const float shx = (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]);
+                for (int oy = -1; oy <= 1; ++oy) {
+                    int nby, iy;
+                    nb_axis(cy, oy, S.nc[1], nby, iy);
+                    const float shy = (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]);
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        int nbz, iz;
+                        nb_axis(cz, oz, S.nc[2], nbz, iz);
+                        const float shz = (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]);
+                        const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+                        const int j0 = p.start[nb], j1 = p.start[nb + 1];
+                        for (int j = j0; j < j1; ++j) {
+                            if (j == s) continue;
+                            const float4 lj = p.lpos[j];
+                            const float dx = (li.x - lj.x) + shx;
+                            const float dy = (li.y - lj.y) + shy;
+                            const float dz = (li.z - lj.z) + shz;
+                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            if (r2 < rc2f) {
+                                if (LAW == LAW_LJ) {
+                                    const float inv = __builtin_amdgcn_rcpf(r2);
+                                    const float t = inv * inv_s2;
+                                    const float s6 = t * t * t;
+                                    const float u = r2 * ur;
+                                    const float vv = __builtin_fmaf(4.0f * s6, s6 - 1.0f,
+                                                                    __builtin_fmaf(be, u - 1.0f, al));
+                                    const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f,
+                                                                    -2.0f * be * u);
+                                    const float fm = ww * inv;
+                                    ax = __builtin_fmaf(fm, dx, ax);
+                                    ay = __builtin_fmaf(fm, dy, ay);
+                                    az = __builtin_fmaf(fm, dz, az);
+                                    pe += (double)vv;
+                                    w += (double)ww;
+                                } else {
+                                    const float nrm = sqrtf(r2);
+                                    const float vv = (float)S.strength * ((float)S.rc - nrm);
+                                    const float ww = (float)S.strength * nrm;
+                                    const float fm = ww / r2;
+                                    ax += fm * dx; ay += fm * dy; az += fm * dz;
+                                    pe += (double)vv;
+                                    w += (double)ww;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (LAW == LAW_LJ && S.eps != 1.0) {
+                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
+                pe *= S.eps; w *= S.eps;
+            }
+            fx = ax; fy = ay; fz = az;
+        }
+    }
+    double acc[4] = {0, 0, 0, 0};
+    if (active) {
+        if (LAW != LAW_NONE) { p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz; }
+        double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+        if (KICK) {
+            const double dt = S.dt;
+            vx = vx + ((1. / 2.) * fx) * dt;
+            vy = vy + ((1. / 2.) * fy) * dt;
+            vz = vz + ((1. / 2.) * fz) * dt;
+            p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+        }
+        const double ddx = p.disp.x[s], ddy = p.disp.y[s], ddz = p.disp.z[s];
+        acc[0] = (vx * vx + vy * vy) + vz * vz;
+        acc[1] = (1. / 2.) * pe;
+        acc[2] = (1. / 2.) * w;
+        acc[3] = (ddx * ddx + ddy * ddy) + ddz * ddz;
+    }
+    double out[4];
+    block_sum4(acc, out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p.partial[4 * b + k] = out[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7. per-system reduction of the block sums into a thermodynamics record
+//    (physics/measurements.hpp:59-70; time += dt as engine/integrator.cpp:85)
+//    mode 0: after a step; 1: after a force-only calculation; 2: measure only
+// ---------------------------------------------------------------------------
+struct FinalArgs {
+    const SysDev* sys;
+    const double* partial;
+    double* time;      // per system
+    double* pe;        // per system (last force calculation)
+    double* vir;
+    int* step;         // per system record counter
+    double* records;   // [cap][nsys][7]
+    double* measure;   // [nsys][7] (mode 2)
+    int nsys;
+    int cap;
+    int mode;
+};
+
+__global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
+{
+    const int k = blockIdx.x;
+    const SysDev& S = f.sys[k];
+    double acc[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < S.nblk; b += kBlock)
+        for (int q = 0; q < 4; ++q) acc[q] += f.partial[4 * (S.blk_first + b) + q];
+    double out[4];
+    block_sum4(acc, out);
+    if (threadIdx.x != 0) return;
+    double pe = f.pe[k], w = f.vir[k];
+    if (f.mode != 2 && S.law != LAW_NONE) {
+        pe = out[1];
+        w = out[2];
+        f.pe[k] = pe;
+        f.vir[k] = w;
+    }
+    double t = f.time[k];
+    if (f.mode == 0) {
+        t = t + S.dt;
+        f.time[k] = t;
+    }
+    const double ke = (1. / 2.) * out[0];
+    double* r;
+    if (f.mode == 0) {
+        int st = f.step[k];
+        f.step[k] = st + 1;
+        if (st >= f.cap) return;
+        r = f.records + 7 * ((size_t)st * f.nsys + k);
+    } else if (f.mode == 2) {
+        r = f.measure + 7 * k;
+    } else {
+        return;
+    }
+    r[0] = t;
+    r[1] = ke;
+    r[2] = pe;
+    r[3] = pe + ke;
+    r[4] = w;
+    r[5] = (2. / 3.) * ke / (double)S.n;
+    r[6] = out[3] / (double)S.n;
+}
+
+__global__ void k_reset_steps(int* step, int nsys)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys) step[k] = 0;
+}
+
+// v *= scale for the slots of one system
+__global__ __launch_bounds__(kBlock) void k_scale_velocities(StepArgs p, int sysk, double scale)
+{
+    const SysDev& S = p.sys[sysk];
+    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    p.vel.x[s] *= scale; p.vel.y[s] *= scale; p.vel.z[s] *= scale;
+}
+
+// ---------------------------------------------------------------------------
+// parity hooks and observers
+// ---------------------------------------------------------------------------
+
+// Cell coordinates of every particle of one system, written by particle id.
+__global__ __launch_bounds__(kBlock) void k_cell_keys(StepArgs p, int sysk, int* keys, int* clamped)
+{
+    const SysDev& S = p.sys[sysk];
+    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    const double r[3] = {p.pos.x[s], p.pos.y[s], p.pos.z[s]};
+    const int i = p.id[s];
+    int nclamp = 0;
+    for (int a = 0; a < 3; ++a) {
+        int k = (int)floor(r[a] * S.q[a]);
+        if (k < 0 || k >= S.nc[a]) ++nclamp;
+        keys[3 * i + a] = cell_axis(r[a], S.q[a], S.nc[a]);
+    }
+    if (nclamp) atomicAdd(clamped, nclamp);
+}
+
+// In-cutoff pairs (first id < second id) of one system.
+__global__ __launch_bounds__(kBlock) void k_pairs_emit(StepArgs p, int sysk, long long cap,
+                                                       int* idx, double* sep,
+                                                       unsigned long long* counter)
+{
+    const SysDev& S = p.sys[sysk];
+    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    const int me = p.id[s];
+    for_each_neighbour_fp64(S, p, s, [&](int j, double dx, double dy, double dz, double,
+                                         int ix, int iy, int iz) {
+        const int other = p.id[j];
+        if (me < other) {
+            unsigned long long k = atomicAdd(counter, 1ull);
+            if ((long long)k < cap) {
+                idx[5 * k] = me; idx[5 * k + 1] = other;
+                idx[5 * k + 2] = ix; idx[5 * k + 3] = iy; idx[5 * k + 4] = iz;
+                sep[3 * k] = dx; sep[3 * k + 1] = dy; sep[3 * k + 2] = dz;
+            }
+        }
+    });
+}
+
+// g(r) histogram: ordered pairs i != j (all images) with r < rmax, per system.
+// Stencil of ceil(rmax / a) cells per axis around the home cell; each
+// (cell, image) combination is visited once even when the stencil wraps.
+__global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bins,
+                                                unsigned long long* hist)
+{
+    extern __shared__ unsigned int lh[];
+    const int b = blockIdx.x;
+    const int k = p.blk_sys[b];
+    const SysDev& S = p.sys[k];
+    for (int t = threadIdx.x; t < bins; t += kBlock) lh[t] = 0;
+    __syncthreads();
+    const int s = b * kBlock + threadIdx.x;
+    if (s - S.atom_off < S.n && S.law != LAW_NONE) {
+        int c[3];
+        cell_coords(p.scell[s] - S.cell_off, S.nc, c[0], c[1], c[2]);
+        int R[3];
+        for (int a = 0; a < 3; ++a) R[a] = (int)ceil(rmax / S.a[a]);
+        const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
+        const double rmax2 = rmax * rmax, dr = rmax / bins;
+        for (int ox = -R[0]; ox <= R[0]; ++ox) {
+            int tx = c[0] + ox;
+            int ix = (tx >= 0) ? tx / S.nc[0] : -((-tx + S.nc[0] - 1) / S.nc[0]);
+            int nbx = tx - ix * S.nc[0];
+            for (int oy = -R[1]; oy <= R[1]; ++oy) {
+                int ty = c[1] + oy;
+                int iy = (ty >= 0) ? ty / S.nc[1] : -((-ty + S.nc[1] - 1) / S.nc[1]);
+                int nby = ty - iy * S.nc[1];
+                for (int oz = -R[2]; oz <= R[2]; ++oz) {
+                    int tz = c[2] + oz;
+                    int iz = (tz >= 0) ? tz / S.nc[2] : -((-tz + S.nc[2] - 1) / S.nc[2]);
+                    int nbz = tz - iz * S.nc[2];
+                    const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+                    const double shx = ix * S.L[0], shy = iy * S.L[1], shz = iz * S.L[2];
+                    for (int j = p.start[nb]; j < p.start[nb + 1]; ++j) {
+                        if (j == s && ix == 0 && iy == 0 && iz == 0) continue;
+                        const double dx = (xi - p.pos.x[j]) - shx;
+                        const double dy = (yi - p.pos.y[j]) - shy;
+                        const double dz = (zi - p.pos.z[j]) - shz;
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        if (r2 < rmax2) {
+                            int bin = (int)(sqrt(r2) / dr);
+                            if (bin < bins) atomicAdd(&lh[bin], 1u);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < bins; t += kBlock)
+        if (lh[t]) atomicAdd(&hist[(size_t)k * bins + t], (unsigned long long)lh[t]);
+}
+
+}  // namespace ljk

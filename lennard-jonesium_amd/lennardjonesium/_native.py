@@ -1,0 +1,115 @@
+"""ctypes bindings of the native libraries built from lennard-jonesium_amd/csrc.
+
+``engine()`` loads liblj_engine.so (HIP kernels + C ABI, include/lj_engine.h)
+and ``host()`` loads liblj_host.so (host C++ layer, include/lj_host.h).  There
+is no fallback: if a library has not been built the call raises, naming the
+build command.
+"""
+
+import ctypes
+import pathlib
+
+PKG_ROOT = pathlib.Path(__file__).resolve().parent.parent
+BUILD_DIR = PKG_ROOT / "build"
+
+_engine = None
+_host = None
+
+c_d = ctypes.c_double
+c_i = ctypes.c_int
+P_d = ctypes.POINTER(ctypes.c_double)
+P_i = ctypes.POINTER(ctypes.c_int)
+
+
+class LJError(RuntimeError):
+    """A native call returned a negative status; the message is lj_last_error()."""
+
+
+class lj_system_desc(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", c_i),
+        ("box", c_d * 3),
+        ("cutoff", c_d),
+        ("time_delta", c_d),
+        ("epsilon", c_d),
+        ("sigma", c_d),
+        ("strength", c_d),
+        ("force_law", c_i),
+        ("periodic", c_i),
+    ]
+
+
+class lj_thermo(ctypes.Structure):
+    _fields_ = [(name, c_d) for name in ("time", "kinetic_energy", "potential_energy",
+                                         "total_energy", "virial", "temperature",
+                                         "mean_square_displacement")]
+
+
+def _load(name):
+    path = BUILD_DIR / name
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build the HIP engine first "
+            "(python lennard-jonesium_amd/build_engine.py or __graft_entry__.build())")
+    return ctypes.CDLL(str(path), mode=ctypes.RTLD_GLOBAL)
+
+
+def engine():
+    global _engine
+    if _engine is not None:
+        return _engine
+    lib = _load("liblj_engine.so")
+    V = ctypes.c_void_p
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "lj_last_error": (ctypes.c_char_p, []),
+        "lj_device_count": (c_i, []),
+        "lj_engine_create": (c_i, [ctypes.POINTER(lj_system_desc), c_i, c_i, PP]),
+        "lj_engine_destroy": (c_i, [V]),
+        "lj_engine_system_count": (c_i, [V]),
+        "lj_engine_set_state": (c_i, [V, c_i, P_d, P_d, P_d, P_d, c_d]),
+        "lj_engine_get_state": (c_i, [V, c_i, P_d, P_d, P_d, P_d, P_d, P_d, P_d]),
+        "lj_engine_step": (c_i, [V, c_i, ctypes.POINTER(lj_thermo)]),
+        "lj_engine_compute_forces": (c_i, [V, P_d, P_d]),
+        "lj_engine_set_temperature": (c_i, [V, c_i, c_d, P_d]),
+        "lj_engine_measure": (c_i, [V, ctypes.POINTER(lj_thermo)]),
+        "lj_engine_cell_keys": (c_i, [V, c_i, P_i, P_i, P_i]),
+        "lj_engine_cell_list": (c_i, [V, c_i, P_i, P_i]),
+        "lj_engine_pairs": (c_i, [V, c_i, ctypes.c_longlong, P_i, P_d,
+                                  ctypes.POINTER(ctypes.c_longlong)]),
+        "lj_engine_rdf_sample": (c_i, [V, c_d, c_i]),
+        "lj_engine_rdf_read": (c_i, [V, P_d, P_i]),
+        "lj_engine_set_graphs": (c_i, [V, c_i]),
+        "lj_engine_set_profiling": (c_i, [V, c_i]),
+        "lj_engine_kernel_times": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_char_p), P_d, P_i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _engine = lib
+    return lib
+
+
+def host():
+    global _host
+    if _host is not None:
+        return _host
+    engine()
+    lib = _load("liblj_host.so")
+    lib.lj_cubic_lattice.restype = c_d
+    lib.lj_cubic_lattice.argtypes = [c_i, c_d, c_i, P_d]
+    lib.lj_initial_condition.restype = c_d
+    lib.lj_initial_condition.argtypes = [c_i, c_d, c_d, ctypes.c_uint, c_i, P_d, P_d]
+    lib.lj_format_double.restype = c_i
+    lib.lj_format_double.argtypes = [c_d, ctypes.c_char_p, c_i]
+    _host = lib
+    return lib
+
+
+def check(status):
+    """Raises LJError with lj_last_error() when status < 0."""
+    if status < 0:
+        msg = engine().lj_last_error()
+        raise LJError((msg or b"").decode() or f"status {status}")
+    return status

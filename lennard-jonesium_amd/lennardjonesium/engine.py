@@ -1,0 +1,236 @@
+"""Python front end of the GPU step engine (include/lj_engine.h).
+
+``Engine`` owns one native engine handle: one or more independent periodic
+systems resident on the GPU.  It is the object the operator-level API
+(``lennardjonesium.operators``) and the orchestration (``run``,
+``run_sweep``) drive; tests use it to reach every C-ABI entry point.
+
+Host arrays are numpy float64 of shape (N, 3), indexed by particle id.
+"""
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from lennardjonesium import _native
+from lennardjonesium._native import check
+
+LAWS = {"lj": 0, "constant": 1, "none": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+THERMO_FIELDS = ("time", "kinetic_energy", "potential_energy", "total_energy", "virial",
+                 "temperature", "mean_square_displacement")
+
+
+@dataclass
+class SystemSpec:
+    """One periodic system (the reference's SystemParameters + force parameters)."""
+
+    particle_count: int
+    box: Sequence[float]
+    cutoff: float = 2.5
+    time_delta: float = 0.005
+    epsilon: float = 1.0
+    sigma: float = 1.0
+    strength: float = 0.0
+    force_law: str = "lj"
+    periodic: bool = True
+
+    def desc(self):
+        d = _native.lj_system_desc()
+        d.particle_count = int(self.particle_count)
+        box = np.broadcast_to(np.asarray(self.box, dtype=np.float64), 3)
+        for a in range(3):
+            d.box[a] = float(box[a])
+        d.cutoff = float(self.cutoff)
+        d.time_delta = float(self.time_delta)
+        d.epsilon = float(self.epsilon)
+        d.sigma = float(self.sigma)
+        d.strength = float(self.strength)
+        d.force_law = LAWS[self.force_law]
+        d.periodic = 1 if self.periodic else 0
+        return d
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_native.P_d) if a is not None else None
+
+
+def _arr(a, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(n, 3))
+    return a
+
+
+class Engine:
+    """GPU engine for ``systems`` (list of SystemSpec), precision 'fp64' or 'mixed'."""
+
+    def __init__(self, systems, precision="fp64", graphs=True):
+        self.lib = _native.engine()
+        self.specs = list(systems)
+        self.precision = precision
+        arr = (_native.lj_system_desc * len(self.specs))(*[s.desc() for s in self.specs])
+        h = ctypes.c_void_p()
+        check(self.lib.lj_engine_create(arr, len(self.specs), PRECISIONS[precision],
+                                        ctypes.byref(h)))
+        self.h = h
+        if not graphs:
+            check(self.lib.lj_engine_set_graphs(self.h, 0))
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lj_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def nsys(self):
+        return len(self.specs)
+
+    # -- state --------------------------------------------------------------
+    def set_state(self, sys, pos, vel, disp=None, force=None, time=0.0):
+        n = self.specs[sys].particle_count
+        pos, vel, disp, force = (_arr(x, n) for x in (pos, vel, disp, force))
+        check(self.lib.lj_engine_set_state(self.h, sys, _dptr(pos), _dptr(vel), _dptr(disp),
+                                           _dptr(force), float(time)))
+
+    def get_state(self, sys):
+        n = self.specs[sys].particle_count
+        out = {k: np.zeros((n, 3)) for k in ("pos", "vel", "disp", "force")}
+        t, pe, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.lib.lj_engine_get_state(self.h, sys, _dptr(out["pos"]), _dptr(out["vel"]),
+                                           _dptr(out["disp"]), _dptr(out["force"]),
+                                           ctypes.byref(t), ctypes.byref(pe), ctypes.byref(w)))
+        out.update(time=t.value, pe=pe.value, vir=w.value)
+        return out
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, nsteps, records=True):
+        """Advances nsteps; returns records of shape (nsteps, nsys, 7) or None."""
+        if records:
+            rec = np.zeros((nsteps, self.nsys, 7))
+            ptr = rec.ctypes.data_as(ctypes.POINTER(_native.lj_thermo))
+        else:
+            rec, ptr = None, None
+        check(self.lib.lj_engine_step(self.h, int(nsteps), ptr))
+        return rec
+
+    def compute_forces(self):
+        pe = np.zeros(self.nsys)
+        w = np.zeros(self.nsys)
+        check(self.lib.lj_engine_compute_forces(self.h, _dptr(pe), _dptr(w)))
+        return pe, w
+
+    def measure(self):
+        out = np.zeros((self.nsys, 7))
+        check(self.lib.lj_engine_measure(self.h, out.ctypes.data_as(
+            ctypes.POINTER(_native.lj_thermo))))
+        return out
+
+    def set_temperature(self, sys, target):
+        m = ctypes.c_double()
+        check(self.lib.lj_engine_set_temperature(self.h, sys, float(target), ctypes.byref(m)))
+        return m.value
+
+    # -- parity hooks ---------------------------------------------------------
+    def cell_keys(self, sys):
+        n = self.specs[sys].particle_count
+        keys = np.zeros((n, 3), dtype=np.int32)
+        nc = np.zeros(3, dtype=np.int32)
+        cl = ctypes.c_int()
+        check(self.lib.lj_engine_cell_keys(self.h, sys, keys.ctypes.data_as(_native.P_i),
+                                           nc.ctypes.data_as(_native.P_i), ctypes.byref(cl)))
+        return keys, nc, cl.value
+
+    def cell_list(self, sys):
+        n = self.specs[sys].particle_count
+        keys, nc, _ = self.cell_keys(sys)
+        start = np.zeros(int(np.prod(nc)) + 1, dtype=np.int32)
+        atoms = np.zeros(n, dtype=np.int32)
+        check(self.lib.lj_engine_cell_list(self.h, sys, start.ctypes.data_as(_native.P_i),
+                                           atoms.ctypes.data_as(_native.P_i)))
+        return start, atoms
+
+    def pairs(self, sys, cap=None):
+        n = self.specs[sys].particle_count
+        cap = int(cap if cap is not None else 64 * n)
+        while True:
+            idx = np.zeros((max(cap, 1), 5), dtype=np.int32)
+            sep = np.zeros((max(cap, 1), 3))
+            cnt = ctypes.c_longlong()
+            check(self.lib.lj_engine_pairs(self.h, sys, cap, idx.ctypes.data_as(_native.P_i),
+                                           _dptr(sep), ctypes.byref(cnt)))
+            if cnt.value <= cap:
+                return idx[:cnt.value], sep[:cnt.value]
+            cap = cnt.value
+
+    # -- observers ------------------------------------------------------------
+    def rdf_sample(self, rmax=4.0, bins=200):
+        check(self.lib.lj_engine_rdf_sample(self.h, float(rmax), int(bins)))
+
+    def rdf_read(self, bins=200):
+        counts = np.zeros((self.nsys, bins))
+        ns = ctypes.c_int()
+        check(self.lib.lj_engine_rdf_read(self.h, _dptr(counts), ctypes.byref(ns)))
+        return counts, ns.value
+
+    # -- tuning / timing ---------------------------------------------------------
+    def set_graphs(self, enabled):
+        check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
+
+    def set_profiling(self, enabled):
+        check(self.lib.lj_engine_set_profiling(self.h, 1 if enabled else 0))
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * 32)()
+        ms = np.zeros(32)
+        cnt = ctypes.c_int()
+        check(self.lib.lj_engine_kernel_times(self.h, 32, names, _dptr(ms), ctypes.byref(cnt)))
+        return {names[k].decode(): float(ms[k]) for k in range(min(cnt.value, 32))}
+
+
+def device_count():
+    return _native.engine().lj_device_count()
+
+
+def initial_condition(n, density, temperature, seed=5489, lattice=2):
+    """engine::InitialCondition on the host (csrc/host/initial_condition.cpp).
+
+    Returns (pos, vel, box_side)."""
+    lib = _native.host()
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    L = lib.lj_initial_condition(int(n), float(density), float(temperature),
+                                 int(seed) & 0xFFFFFFFF, int(lattice), _dptr(pos), _dptr(vel))
+    if L < 0:
+        raise ValueError("invalid initial-condition parameters")
+    return pos, vel, L
+
+
+def cubic_lattice(n, density, lattice=2):
+    lib = _native.host()
+    pos = np.zeros((n, 3))
+    L = lib.lj_cubic_lattice(int(n), float(density), int(lattice), _dptr(pos))
+    if L < 0:
+        raise ValueError("invalid lattice parameters")
+    return pos, L
+
+
+def format_double(x):
+    buf = ctypes.create_string_buffer(64)
+    _native.host().lj_format_double(float(x), buf, 64)
+    return buf.value.decode()

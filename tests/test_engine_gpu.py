@@ -1,0 +1,312 @@
+"""Parity of the HIP engine (through the C ABI) against the CPU oracle.
+
+Bars (written where they are checked):
+  * integer work -- cell keys, cell lists, pair sets: bit-exact;
+  * FP64 forces / PE / virial on identical positions: 1e-12 relative
+    (force norm-wise: max_i |F_i - F_i^ref| / max(|F^ref|_rms, |F_i^ref|));
+  * trajectories: track the oracle / the reference fixture until chaos;
+  * mixed precision: stated error bounds against the FP64 oracle.
+"""
+
+import csv
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ljref
+
+pytestmark = pytest.mark.gpu
+
+FTOL = 1e-12
+
+
+def _engine_mod():
+    from lennardjonesium import engine as eng
+    assert eng.device_count() > 0, "no HIP device"
+    return eng
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def perturbed_lattice(n, rho, amp, seed):
+    pos, L = ljref.cubic_lattice(n, rho)
+    rng = np.random.default_rng(seed)
+    pos = ljref.wrap(pos + rng.uniform(-amp, amp, size=pos.shape), L)
+    return pos, L
+
+
+def force_err(F, Fref):
+    rms = np.sqrt((Fref ** 2).sum(1).mean())
+    den = np.maximum(np.linalg.norm(Fref, axis=1), rms)
+    return (np.linalg.norm(F - Fref, axis=1) / den).max()
+
+
+def make(eng, n, L, precision="fp64", **kw):
+    e = eng.Engine([eng.SystemSpec(n, L, **kw)], precision=precision)
+    return e
+
+
+# ---- cell keys / cell lists / pair sets: bit-exact -------------------------------
+
+@pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.3), (4000, 0.8, 0.5), (500, 1.2, 0.4),
+                                       (32000, 0.05, 1.0)])
+def test_cell_keys_and_lists_bit_exact(n, rho, amp):
+    eng = _engine_mod()
+    _say("cell keys", n, rho)
+    pos, L = perturbed_lattice(n, rho, amp, n)
+    # exact boundary values: 0, just below L, and the wrap's rounding cases
+    pos[0] = [0.0, np.nextafter(L, 0), L / 2]
+    pos[1] = [np.nextafter(0.0, 1), L * (1 - 1e-16), 0.0]
+    e = make(eng, n, L)
+    e.set_state(0, pos, np.zeros_like(pos))
+    keys, nc, clamped = e.cell_keys(0)
+    rkeys, rclamped = ljref.cell_keys(pos, L, 2.5)
+    assert np.array_equal(keys, rkeys) and clamped == rclamped
+    assert list(nc) == list(ljref.cell_shape(L, 2.5))
+    start, atoms = e.cell_list(0)
+    rstart, ratoms = ljref.cell_list(pos, L, 2.5)
+    assert np.array_equal(start, rstart) and np.array_equal(atoms, ratoms)
+    e.close()
+
+
+def _pairset(idx, sep):
+    """Orientation-free pair set {(i, j, image)} with i < j (bit-exact keys)."""
+    out = set()
+    for (i, j, ix, iy, iz), s in zip(idx.tolist(), sep.tolist()):
+        if i > j:
+            i, j, ix, iy, iz = j, i, -ix, -iy, -iz
+            s = [-x for x in s]
+        out.add((i, j, ix, iy, iz, *(float(x) for x in s)))
+    return out
+
+
+@pytest.mark.parametrize("n,L,rc", [(10, 10.0, 2.0), (864, None, 2.5), (60, 5.0, 2.5),
+                                    (40, 3.0, 2.5), (500, None, 2.5)])
+def test_pair_set_bit_exact(n, L, rc):
+    eng = _engine_mod()
+    _say("pairs", n, L, rc)
+    if n == 10:   # tests/cpp/lennardjonesium/engine/test_particle_pair_filter.cpp:59-125
+        pos = np.array([[0, 0, 0], [1, 1, 1], [1, 9, 9], [9, 1, 9], [9, 9, 1], [5, 5, 5],
+                        [4, 4, 4], [4, 6, 6], [6, 4, 6], [6, 6, 4]], dtype=float)
+    elif L is None:
+        pos, L = perturbed_lattice(n, 0.8 if n == 864 else 1.2, 0.3, 7)
+    else:
+        pos = np.random.default_rng(n).uniform(0, L, size=(n, 3))
+    e = eng.Engine([eng.SystemSpec(n, L, cutoff=rc)])
+    e.set_state(0, pos, np.zeros_like(pos))
+    got = _pairset(*e.pairs(0))
+    want = _pairset(*ljref.pairs_cell(pos, L, rc))
+    assert got == want and len(want) > 0
+    e.close()
+
+
+# ---- forces, PE, virial: FP64 within 1e-12 ------------------------------------------
+
+@pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.2), (4000, 0.8, 0.3), (500, 1.2, 0.2),
+                                       (256000, 0.8442, 0.15)])
+def test_forces_fp64(n, rho, amp):
+    eng = _engine_mod()
+    _say("forces", n)
+    pos, L = perturbed_lattice(n, rho, amp, 11)
+    e = make(eng, n, L)
+    e.set_state(0, pos, np.zeros_like(pos))
+    pe, w = e.compute_forces()
+    st = e.get_state(0)
+    Fr, per, wr = ljref.forces(pos, L)
+    assert force_err(st["force"], Fr) < FTOL
+    assert abs(pe[0] - per) / abs(per) < FTOL
+    assert abs(w[0] - wr) / abs(wr) < FTOL
+    assert np.array_equal(st["pos"], pos)          # forces do not move particles
+    e.close()
+
+
+@pytest.mark.parametrize("z", [(0.2, 0.6), (0.8, 1.2), (2.8, 0.2)])
+def test_mock_force_known_answers(z):
+    # tests/cpp/lennardjonesium/engine/test_short_range_force_calculation.cpp:39-109
+    eng = _engine_mod()
+    pos = np.array([[0.2, 0.2, z[0]], [0.2, 0.2, z[1]]])
+    e = eng.Engine([eng.SystemSpec(2, 3.0, cutoff=1.0, force_law="constant", strength=-10.0)])
+    e.set_state(0, pos, np.zeros_like(pos))
+    pe, w = e.compute_forces()
+    np.testing.assert_allclose(e.get_state(0)["force"], [[0, 0, 10], [0, 0, -10]], atol=1e-12)
+    assert pe[0] == pytest.approx(-6.0) and w[0] == pytest.approx(-4.0)
+    e.close()
+
+
+def test_mock_force_l_shape():
+    eng = _engine_mod()
+    pos = np.array([[0.6, 0.6, 0.2], [1.4, 0.6, 0.2], [0.6, 1.4, 0.2]])
+    e = eng.Engine([eng.SystemSpec(3, 3.0, cutoff=1.0, force_law="constant", strength=-10.0)])
+    e.set_state(0, pos, np.zeros_like(pos))
+    pe, w = e.compute_forces()
+    np.testing.assert_allclose(e.get_state(0)["force"], [[10, 10, 0], [-10, 0, 0], [0, -10, 0]],
+                               atol=1e-12)
+    assert pe[0] == pytest.approx(-4.0) and w[0] == pytest.approx(-16.0)
+    e.close()
+
+
+# ---- integrator known answers (dt = 1): test_velocity_verlet_integrator.cpp -------------
+
+def test_vv_known_answers():
+    eng = _engine_mod()
+    vel = np.array([[1.0, 0, 0], [0, 1.0, 0]])
+    e = eng.Engine([eng.SystemSpec(2, 1.0, time_delta=1.0, force_law="none", periodic=False)])
+    e.set_state(0, np.zeros((2, 3)), vel)
+    rec = e.step(4)
+    st = e.get_state(0)
+    assert st["pos"].tolist() == [[4, 0, 0], [0, 4, 0]] and st["disp"].tolist() == st["pos"].tolist()
+    assert st["vel"].tolist() == vel.tolist() and st["time"] == 4.0
+    assert rec[:, 0, 0].tolist() == [1.0, 2.0, 3.0, 4.0]
+    e.set_state(0, np.zeros((2, 3)), vel, force=[[0, 0, -1.0]] * 2)
+    e.step(4)
+    st = e.get_state(0)
+    z = -(0.5 + 1.5 + 2.5 + 3.5)
+    assert st["pos"].tolist() == [[4, 0, z], [0, 4, z]]
+    assert st["vel"].tolist() == [[1, 0, -4], [0, 1, -4]]
+    e.close()
+    e = eng.Engine([eng.SystemSpec(2, 3.0, time_delta=1.0, force_law="none", periodic=True)])
+    e.set_state(0, np.zeros((2, 3)), vel)
+    e.step(4)
+    st = e.get_state(0)
+    assert st["pos"].tolist() == [[1, 0, 0], [0, 1, 0]]
+    assert st["disp"].tolist() == [[4, 0, 0], [0, 4, 0]]
+    e.close()
+
+
+# ---- trajectories ---------------------------------------------------------------------------
+
+def _fixture_rows(golden):
+    rows = list(csv.reader(open(golden / "quick_single_run/thermodynamics_head.csv")))
+    return np.array([[float(x) for x in r] for r in rows[1:]])
+
+
+def test_trajectory_tracks_reference_fixture(golden):
+    """notebooks/quick_single_run (reference binary): 200 steps incl. two rescales."""
+    eng = _engine_mod()
+    ref = _fixture_rows(golden)
+    pos, vel, L = eng.initial_condition(500, 0.8, 0.5, seed=117208264)
+    e = make(eng, 500, L)
+    e.set_state(0, pos, vel)
+    got = []
+    for chunk in range(4):
+        rec = e.step(50)[:, 0, :]
+        got.append(rec)
+        if chunk < 2:   # events.log: rescale after steps 50 and 100
+            T = rec[:, 5].mean()
+            e.set_temperature(0, 0.5)
+    got = np.concatenate(got)
+    rel = np.abs(got - ref[:, 1:]) / np.abs(ref[:, 1:])
+    _say("fixture max rel", rel.max())
+    assert rel[:50].max() < 1e-12 and rel.max() < 1e-10
+    e.close()
+
+
+def test_trajectory_tracks_oracle_and_graphs_are_exact():
+    eng = _engine_mod()
+    st, L = ljref.initial_condition(864, 0.8, 1.0, seed=5489)
+    ref = ljref.run(st.copy(), L, 0.005, 400)
+    out = []
+    for graphs in (True, False):
+        e = eng.Engine([eng.SystemSpec(864, L)], graphs=graphs)
+        e.set_state(0, st.pos, st.vel)
+        out.append(e.step(400)[:, 0, :])
+        e.close()
+    assert np.array_equal(out[0], out[1])       # graph replay == eager launches, bit for bit
+    rel = np.abs(out[0] - ref) / np.abs(ref)
+    _say("oracle max rel", rel[:100].max(), rel.max())
+    assert rel[:100].max() < 1e-11 and rel.max() < 1e-6
+
+
+def test_batched_replicas_match_single_runs():
+    eng = _engine_mod()
+    cases = [(500, 0.8, 0.7, 3), (256, 0.3, 1.5, 4), (864, 1.1, 0.9, 5)]
+    specs, states = [], []
+    for n, rho, T, seed in cases:
+        st, L = ljref.initial_condition(n, rho, T, seed=seed)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((st, L))
+    e = eng.Engine(specs)
+    for k, (st, _) in enumerate(states):
+        e.set_state(k, st.pos, st.vel)
+    rec = e.step(100)
+    for k, (st, L) in enumerate(states):
+        single = eng.Engine([specs[k]])
+        single.set_state(0, st.pos, st.vel)
+        r1 = single.step(100)[:, 0, :]
+        assert np.array_equal(rec[:, k, :], r1)      # batching does not change the arithmetic
+        ref = ljref.run(st.copy(), L, 0.005, 100)
+        assert (np.abs(r1 - ref) / np.abs(ref)).max() < 1e-10
+        single.close()
+    e.close()
+
+
+def test_set_temperature_and_measure():
+    eng = _engine_mod()
+    st, L = ljref.initial_condition(864, 0.8, 1.0, seed=5489)
+    e = make(eng, 864, L)
+    e.set_state(0, st.pos, st.vel * 1.3)
+    measured = e.set_temperature(0, 0.7)
+    assert measured == pytest.approx(1.69, rel=1e-12)
+    m = e.measure()[0]
+    assert m[5] == pytest.approx(0.7, rel=1e-13)
+    e.close()
+
+
+# ---- mixed precision: error stated against the FP64 oracle -------------------------------
+
+@pytest.mark.parametrize("n,rho", [(4000, 0.8), (256000, 0.8442)])
+def test_mixed_forces_error(n, rho):
+    eng = _engine_mod()
+    pos, L = perturbed_lattice(n, rho, 0.15, 3)
+    e = make(eng, n, L, precision="mixed")
+    e.set_state(0, pos, np.zeros_like(pos))
+    pe, w = e.compute_forces()
+    F = e.get_state(0)["force"]
+    Fr, per, wr = ljref.forces(pos, L)
+    rms_rel = np.sqrt(((F - Fr) ** 2).sum(1).mean() / (Fr ** 2).sum(1).mean())
+    _say("mixed force rms rel", rms_rel, "pe rel", abs(pe[0] - per) / abs(per))
+    assert rms_rel < 1e-5
+    assert abs(pe[0] - per) / abs(per) < 1e-5 and abs(w[0] - wr) / abs(wr) < 1e-5
+    e.close()
+
+
+def energy_drift(E, window=100):
+    """Secular drift |<E>_end - <E>_start| / |E_0| over windows of `window` steps
+    (separates drift from the O(dt^2) fluctuation of velocity Verlet)."""
+    return abs(E[-window:].mean() - E[:window].mean()) / abs(E[0])
+
+
+def test_mixed_energy_drift_matches_fp64():
+    eng = _engine_mod()
+    pos, vel, L = eng.initial_condition(4000, 0.8, 1.0, seed=5489)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        e = make(eng, 4000, L, precision=prec)
+        e.set_state(0, pos, vel)
+        E = e.step(2000)[:, 0, 3]
+        out[prec] = (energy_drift(E), np.abs(E - E[0]).max() / abs(E[0]))
+        e.close()
+    _say("drift / max fluctuation", out)
+    # the fluctuation is the integrator's (same in both precisions); drift < 1e-4
+    assert out["mixed"][0] < 1e-4 and out["fp64"][0] < 1e-4
+    assert out["mixed"][1] < 1.2 * out["fp64"][1] + 1e-5
+
+
+# ---- g(r) ------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,rho", [(500, 1.2), (256, 0.3)])
+def test_rdf_histogram_matches_brute_force(n, rho):
+    eng = _engine_mod()
+    pos, L = perturbed_lattice(n, rho, 0.3, 5)
+    e = make(eng, n, L)
+    e.set_state(0, pos, np.zeros_like(pos))
+    e.rdf_sample(4.0, 200)
+    counts, ns = e.rdf_read(200)
+    ref = ljref.rdf_hist(pos, L, 4.0, 200)
+    assert ns == 1
+    assert np.array_equal(counts[0], ref)
+    e.close()

@@ -1,0 +1,70 @@
+"""CPU-side checks of the native libraries: they load, export every symbol
+the C headers declare, and the host-layer routines match the oracle."""
+
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+
+def _declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lj_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("lj_engine.h", "engine"), ("lj_host.h", "host")])
+def test_library_exports_every_declared_symbol(header, lib):
+    from lennardjonesium import _native
+    so = getattr(_native, lib)()
+    names = _declared(header)
+    assert names
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_engine_create_without_device_fails_cleanly():
+    from lennardjonesium import _native
+    lib = _native.engine()
+    if lib.lj_device_count() > 0:
+        pytest.skip("a device is visible")
+    from lennardjonesium.engine import Engine, SystemSpec
+    with pytest.raises(_native.LJError):
+        Engine([SystemSpec(10, 5.0)])
+
+
+@pytest.mark.parametrize("n,rho,T,seed,lat", [(500, 0.8, 0.5, 117208264, 2), (864, 0.8, 1.0, 5489, 2),
+                                              (100, 0.8, 0.3, 5489, 2), (7, 7 / 8, 0.5, 1, 0),
+                                              (54, 0.5, 1.2, 99, 1)])
+def test_host_initial_condition_matches_oracle(n, rho, T, seed, lat):
+    from oracle import ljref
+    from lennardjonesium.engine import initial_condition
+    st, L0 = ljref.initial_condition(n, rho, T, seed=seed, cell_type=lat)
+    pos, vel, L = initial_condition(n, rho, T, seed=seed, lattice=lat)
+    assert L == L0
+    assert np.array_equal(pos, st.pos)
+    np.testing.assert_allclose(vel, st.vel, rtol=0, atol=1e-13 * np.abs(st.vel).max())
+
+
+def test_format_double_matches_reference_sinks():
+    # tests/cpp/lennardjonesium/output/test_sinks.cpp:100-225 and the reference's
+    # notebook CSVs (fmt "{}" output)
+    from lennardjonesium.engine import format_double
+    cases = {3.5: "3.5", 2.25: "2.25", 0.5: "0.5", 7.25: "7.25", 0.0: "0", 2.0: "2",
+             9.0: "9", -4.0: "-4", 3.7500000000000003e-05: "3.7500000000000003e-05",
+             2.9999999999999583: "2.9999999999999583", -10810.337076759344: "-10810.337076759344",
+             0.0001498634676421689: "0.0001498634676421689", 1e16: "1e+16",
+             1234567890123456.0: "1234567890123456", 0.005: "0.005", 1e-5: "1e-05",
+             -2.5e-300: "-2.5e-300", 0.2500000000000001: "0.2500000000000001"}
+    for x, want in cases.items():
+        assert format_double(x) == want, (x, format_double(x), want)
+
+
+def test_format_double_reproduces_fixture_lines(golden):
+    from lennardjonesium.engine import format_double
+    lines = (golden / "quick_single_run/thermodynamics_head.csv").read_text().splitlines()[1:]
+    for line in lines:
+        fields = line.split(",")
+        assert ",".join([fields[0]] + [format_double(float(x)) for x in fields[1:]]) == line
