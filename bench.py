@@ -42,8 +42,8 @@ def pair_flops(c_half, p_half):
     return 8.0 * c_half + 30.0 * p_half
 
 
-def run_gpu(eng, precision, pos, vel, L, steps, warmup, n):
-    e = eng.Engine([eng.SystemSpec(n, L)], precision=precision)
+def run_gpu(eng, precision, pos, vel, L, steps, warmup, n, pair_kernel="tiled"):
+    e = eng.Engine([eng.SystemSpec(n, L)], precision=precision, pair_kernel=pair_kernel)
     e.set_state(0, pos, vel)
     if warmup:
         e.step(warmup)
@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--pair-kernel", default="tiled", choices=["tiled", "particle"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs every system on a single "
@@ -113,7 +114,8 @@ def main():
     pos, vel, L = eng.initial_condition(n, rho, T, seed=seed)
     log(f"workload N={n} L={L:.6f} steps={args.steps} warmup={args.warmup}")
 
-    em, recm, tm = run_gpu(eng, "mixed", pos, vel, L, args.steps, args.warmup, n)
+    em, recm, tm = run_gpu(eng, "mixed", pos, vel, L, args.steps, args.warmup, n,
+                           args.pair_kernel)
     rate_m = n * args.steps / tm
     Em = recm[:, 3]
     drift_m = float(np.abs(Em - Em[0]).max() / abs(Em[0]))
@@ -121,7 +123,8 @@ def main():
 
     res64 = None
     if not args.no_fp64:
-        e64, rec64, t64 = run_gpu(eng, "fp64", pos, vel, L, args.steps, args.warmup, n)
+        e64, rec64, t64 = run_gpu(eng, "fp64", pos, vel, L, args.steps, args.warmup, n,
+                                  args.pair_kernel)
         rate_64 = n * args.steps / t64
         E64 = rec64[:, 3]
         drift_64 = float(np.abs(E64 - E64[0]).max() / abs(E64[0]))
@@ -188,7 +191,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
                      "traffic": traffic,
-                     "kernel": "k_pair<MIXED,LJ,KICK>",
+                     "kernel": ("k_pair_cells<MIXED,LJ,KICK>" if args.pair_kernel == "tiled"
+                                else "k_pair<MIXED,LJ,KICK>"),
                      "kernel_ms": ktm["pair"],
                      "flops_per_launch": flops, "c_half": c_half, "p_half": p_half},
         "kernel_ms": ktm,

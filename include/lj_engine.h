@@ -158,6 +158,12 @@ int lj_engine_cell_list(lj_engine* e, int sys, int* start, int* atoms);
 int lj_engine_pairs(lj_engine* e, int sys, long long cap, int* idx, double* sep,
                     long long* count);
 
+/* Number of in-cutoff neighbours (pairs with each image counted) of every
+ * particle of system `sys` as decided by the production pair kernel (the
+ * current precision and pair-kernel variant) on the current positions;
+ * counts: N ints in particle-id order.  Also recomputes the forces. */
+int lj_engine_neighbor_counts(lj_engine* e, int sys, int* counts);
+
 /* Ordered-pair distance histogram of every system on the current positions
  * (bins of width rmax / bins over [0, rmax), images included); accumulates
  * into the engine's g(r) buffer.  lj_engine_rdf_read copies the counts
@@ -169,6 +175,14 @@ int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples);
 
 /* 0 disables HIP graph capture of the step loop (launches each kernel). */
 int lj_engine_set_graphs(lj_engine* e, int enabled);
+
+/* Pair-kernel variant: LJ_PAIR_CELL_TILED (default) runs one wave64 per home
+ * cell with the neighbour cells staged in LDS; LJ_PAIR_PER_PARTICLE runs one
+ * thread per particle over the 27 neighbour cells.  Both give the same pair
+ * set and, in FP64, the same per-pair arithmetic. */
+#define LJ_PAIR_PER_PARTICLE 0
+#define LJ_PAIR_CELL_TILED 1
+int lj_engine_set_pair_kernel(lj_engine* e, int variant);
 
 /* Mean duration in milliseconds of each step-loop kernel over the last
  * lj_engine_step call with profiling enabled (HIP events on the engine's

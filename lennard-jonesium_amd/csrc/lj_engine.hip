@@ -67,9 +67,13 @@ struct lj_engine {
     std::vector<SysDev> sys;
     int nslot = 0, nblk = 0;
     int ncell_total = 0, ntile = 0;
+    int ncblk = 0;                  // cell-space blocks (tiled pair kernel)
+    int pair_variant = LJ_PAIR_CELL_TILED;
+    bool last_pair_cells = false;   // partial sums of the last pair launch are cell-space
 
     SysDev* d_sys = nullptr;
     int* d_blk_sys = nullptr;
+    int* d_cblk_sys = nullptr;
     double* d_pos[2] = {nullptr, nullptr};
     double* d_vel[2] = {nullptr, nullptr};
     double* d_disp[2] = {nullptr, nullptr};
@@ -88,6 +92,7 @@ struct lj_engine {
     int rdf_bins = 0, rdf_samples = 0;
     double rdf_rmax = 0;
 
+    int* d_ncount = nullptr;    // set only while a neighbour-count hook runs
     int cur = 0;                // which buffer set holds the current state
     bool sorted = false;        // slots are in cell order (after a cell build)
     hipStream_t stream = nullptr;
@@ -104,6 +109,7 @@ struct lj_engine {
         ljk::StepArgs p;
         p.sys = d_sys;
         p.blk_sys = d_blk_sys;
+        p.cblk_sys = d_cblk_sys;
         auto planes = [&](double* base) {
             return ljk::Planes{base, base + nslot, base + 2 * (size_t)nslot};
         };
@@ -126,6 +132,7 @@ struct lj_engine {
         p.scell = d_scell;
         p.lpos = d_lpos;
         p.partial = d_partial;
+        p.ncount = d_ncount;
         p.ncell_total = ncell_total;
         p.nslot = nslot;
         return p;
@@ -145,6 +152,7 @@ struct lj_engine {
         f.nsys = nsys;
         f.cap = rec_cap;
         f.mode = mode;
+        f.cell_blocks = last_pair_cells ? 1 : 0;
         return f;
     }
 };
@@ -222,32 +230,43 @@ void launch_pair(lj_engine* e, const ljk::StepArgs& p)
                        p);
 }
 
+template <int PREC, int LAW, bool KICK>
+void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
+{
+    hipLaunchKernelGGL((ljk::k_pair_cells<PREC, LAW, KICK>), dim3(e->ncblk), dim3(kBlock), 0,
+                       e->stream, p);
+}
+
+template <int LAW, bool KICK>
+void dispatch_pair(lj_engine* e, const ljk::StepArgs& p, bool mixed, bool cells)
+{
+    if (cells) {
+        if (mixed) launch_pair_cells<ljk::MIXED, LAW, KICK>(e, p);
+        else launch_pair_cells<ljk::FP64, LAW, KICK>(e, p);
+    } else {
+        if (mixed) launch_pair<ljk::MIXED, LAW, KICK>(e, p);
+        else launch_pair<ljk::FP64, LAW, KICK>(e, p);
+    }
+}
+
 // Pair forces (or, for LJ_LAW_NONE engines / measure, just the sums).
 int enqueue_pair(lj_engine* e, bool kick, bool measure_only)
 {
     const ljk::StepArgs p = e->args(e->cur);
     const int law = measure_only ? ljk::LAW_NONE : e->sys[0].law;
     const bool mixed = e->precision == LJ_PRECISION_MIXED;
+    const bool cells = law != ljk::LAW_NONE && e->pair_variant == LJ_PAIR_CELL_TILED;
+    e->last_pair_cells = cells;
     return timed(e, 5, "pair", [&] {
         if (law == ljk::LAW_NONE) {
             if (kick) launch_pair<ljk::FP64, ljk::LAW_NONE, true>(e, p);
             else launch_pair<ljk::FP64, ljk::LAW_NONE, false>(e, p);
         } else if (law == ljk::LAW_LJ) {
-            if (mixed) {
-                if (kick) launch_pair<ljk::MIXED, ljk::LAW_LJ, true>(e, p);
-                else launch_pair<ljk::MIXED, ljk::LAW_LJ, false>(e, p);
-            } else {
-                if (kick) launch_pair<ljk::FP64, ljk::LAW_LJ, true>(e, p);
-                else launch_pair<ljk::FP64, ljk::LAW_LJ, false>(e, p);
-            }
+            if (kick) dispatch_pair<ljk::LAW_LJ, true>(e, p, mixed, cells);
+            else dispatch_pair<ljk::LAW_LJ, false>(e, p, mixed, cells);
         } else {
-            if (mixed) {
-                if (kick) launch_pair<ljk::MIXED, ljk::LAW_CONSTANT, true>(e, p);
-                else launch_pair<ljk::MIXED, ljk::LAW_CONSTANT, false>(e, p);
-            } else {
-                if (kick) launch_pair<ljk::FP64, ljk::LAW_CONSTANT, true>(e, p);
-                else launch_pair<ljk::FP64, ljk::LAW_CONSTANT, false>(e, p);
-            }
+            if (kick) dispatch_pair<ljk::LAW_CONSTANT, true>(e, p, mixed, cells);
+            else dispatch_pair<ljk::LAW_CONSTANT, false>(e, p, mixed, cells);
         }
     });
 }
@@ -378,7 +397,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             return fail(LJ_E_INVALID, "all systems of an engine must use the same force law");
         }
 
-    int atom_off = 0, cell_off = 0, blk = 0;
+    int atom_off = 0, cell_off = 0, blk = 0, cblk = 0;
     std::vector<int> count_init;
     for (int k = 0; k < nsys; ++k) {
         const lj_system_desc& d = systems[k];
@@ -411,6 +430,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             }
             // physics/lennard_jones_force.cpp:68-73 on the reduced cutoff
             s.rc2_red = d.cutoff * d.cutoff;
+            s.inv_rc2s = 1.0 / (s.sigma2 * s.rc2_red);
             double rcm6 = 1 / (s.rc2_red * s.rc2_red * s.rc2_red);
             s.alpha = -4.0 * rcm6 * (rcm6 - 1.0);
             s.beta = 12.0 * rcm6 * (2.0 * rcm6 - 1.0);
@@ -436,6 +456,9 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         s.atom_off = atom_off;
         s.blk_first = blk;
         s.nblk = slots / kBlock;
+        s.cblk_first = cblk;
+        s.ncblk = std::max(1, (s.ncell + ljk::kCellsPerBlock - 1) / ljk::kCellsPerBlock);
+        cblk += s.ncblk;
         if (e->has_cells) {
             s.cell_off = cell_off;
             count_init.resize(cell_off + s.ncell + 1, 0);
@@ -448,12 +471,15 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     }
     e->nslot = atom_off;
     e->nblk = blk;
+    e->ncblk = cblk;
     e->ncell_total = cell_off;
     e->ntile = std::max(1, (cell_off + kScanTile - 1) / kScanTile);
 
-    std::vector<int> blk_sys(blk);
-    for (int k = 0; k < nsys; ++k)
+    std::vector<int> blk_sys(blk), cblk_sys(cblk);
+    for (int k = 0; k < nsys; ++k) {
         for (int b = 0; b < e->sys[k].nblk; ++b) blk_sys[e->sys[k].blk_first + b] = k;
+        for (int b = 0; b < e->sys[k].ncblk; ++b) cblk_sys[e->sys[k].cblk_first + b] = k;
+    }
 
     auto cleanup_fail = [&](int rc) {
         lj_engine_destroy(e);
@@ -466,7 +492,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         return cleanup_fail(fail(LJ_E_HIP, "hipEventCreate failed"));
     const size_t ns = e->nslot;
     if ((rc = dev_alloc(&e->d_sys, nsys)) || (rc = dev_alloc(&e->d_blk_sys, blk)) ||
-        (rc = dev_alloc(&e->d_force, 3 * ns)) || (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
+        (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_force, 3 * ns)) ||
+        (rc = dev_alloc(&e->d_partial, 4 * (size_t)std::max(blk, cblk))) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
         (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
@@ -498,6 +525,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             ident[e->sys[k].atom_off + i] = i < e->sys[k].n ? i : 0;
     bool ok = h2d(e->d_sys, e->sys.data(), nsys * sizeof(SysDev)) &&
               h2d(e->d_blk_sys, blk_sys.data(), blk * sizeof(int)) &&
+              h2d(e->d_cblk_sys, cblk_sys.data(), cblk * sizeof(int)) &&
               h2d(e->d_force, zeros.data(), 3 * ns * sizeof(double)) &&
               h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
@@ -523,7 +551,7 @@ int lj_engine_destroy(lj_engine* e)
     if (!e) return LJ_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
-    void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
+    void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_start, e->d_tile,
                     e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_records,
                     e->d_measure, e->d_rdf};
@@ -787,6 +815,33 @@ int lj_engine_pairs(lj_engine* e, int sys, long long cap, int* idx, double* sep,
     return LJ_OK;
 }
 
+int lj_engine_neighbor_counts(lj_engine* e, int sys, int* counts)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    const SysDev& S = e->sys[sys];
+    if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
+    if ((rc = dev_alloc(&e->d_ncount, e->nslot))) return rc;
+    rc = enqueue_cells(e, false);
+    if (!rc) rc = enqueue_pair(e, false, false);
+    if (!rc) rc = enqueue_finalize(e, 1);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    std::vector<int> cnt(S.n), ids(S.n);
+    if (he == hipSuccess)
+        he = hipMemcpy(cnt.data(), e->d_ncount + S.atom_off, S.n * sizeof(int), hipMemcpyDeviceToHost);
+    if (he == hipSuccess)
+        he = hipMemcpy(ids.data(), e->d_id[e->cur] + S.atom_off, S.n * sizeof(int),
+                       hipMemcpyDeviceToHost);
+    (void)hipFree(e->d_ncount);
+    e->d_ncount = nullptr;
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(LJ_E_HIP, std::string("neighbor_counts: ") + hipGetErrorString(he));
+    if (counts)
+        for (int s = 0; s < S.n; ++s) counts[ids[s]] = cnt[s];
+    return LJ_OK;
+}
+
 int lj_engine_rdf_sample(lj_engine* e, double rmax, int bins)
 {
     int rc = check_engine(e);
@@ -837,6 +892,20 @@ int lj_engine_set_graphs(lj_engine* e, int enabled)
     int rc = check_engine(e);
     if (rc) return rc;
     e->use_graphs = enabled != 0;
+    return LJ_OK;
+}
+
+int lj_engine_set_pair_kernel(lj_engine* e, int variant)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED)
+        return fail(LJ_E_INVALID, "unknown pair kernel variant");
+    if (variant != e->pair_variant && e->graph2) {
+        HIP_TRY(hipGraphExecDestroy(e->graph2));
+        e->graph2 = nullptr;
+    }
+    e->pair_variant = variant;
     return LJ_OK;
 }
 

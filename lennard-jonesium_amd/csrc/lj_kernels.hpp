@@ -30,6 +30,7 @@ namespace ljk {
 
 constexpr int kBlock = 256;        // slots per block in particle-space kernels
 constexpr int kScanTile = 1024;    // cells per block in the scan kernels
+constexpr int kCellsPerBlock = 4;  // one wave64 per home cell in the tiled pair kernel
 
 enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
 enum Prec { FP64 = 0, MIXED = 1 };
@@ -45,12 +46,14 @@ struct SysDev {
     int periodic;
     int blk_first;     // first particle-space block
     int nblk;          // blocks of this system
-    int pad_;
+    int cblk_first;    // first cell-space block (kCellsPerBlock cells each)
+    int ncblk;         // cell-space blocks of this system
     double L[3];       // box
     double q[3];       // nc / L (the reference forms this quotient first)
     double a[3];       // L / nc (cell side, used for cell-local coordinates)
     double rc2;        // absolute filter cutoff squared
     double rc2_red;    // LJ reduced cutoff squared
+    double inv_rc2s;   // 1 / (sigma^2 rc2_red): (r / rc)^2 = r^2 * inv_rc2s
     double alpha, beta;
     double eps, sigma2;
     double strength, rc;   // constant law
@@ -128,6 +131,7 @@ struct Planes {       // three coordinate planes of one per-slot vector field
 struct StepArgs {
     const SysDev* sys;
     const int* blk_sys;
+    const int* cblk_sys;                // cell-space block -> system
     Planes pos, vel, disp, force;       // current buffers
     Planes pos_n, vel_n, disp_n;        // next buffers (sort destination)
     int* id;                            // current ids
@@ -142,6 +146,7 @@ struct StepArgs {
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per block: sum v^2, PE, W, sum d^2
+    int* ncount;                        // optional: in-cutoff neighbours per slot
     int ncell_total;
     int nslot;
 };
@@ -156,35 +161,57 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
     const int b = blockIdx.x;
     const SysDev& S = p.sys[p.blk_sys[b]];
     const int s = b * kBlock + threadIdx.x;
-    if (s - S.atom_off >= S.n) return;
-    double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
-    if (INTEGRATE) {
-        const double dt = S.dt;
-        // v += (1/2) F dt   (the stored force: zero before the first calculation)
-        double vx = p.vel.x[s] + ((1. / 2.) * p.force.x[s]) * dt;
-        double vy = p.vel.y[s] + ((1. / 2.) * p.force.y[s]) * dt;
-        double vz = p.vel.z[s] + ((1. / 2.) * p.force.z[s]) * dt;
-        // r += v dt ; d += v dt  (same increment)
-        double ix = vx * dt, iy = vy * dt, iz = vz * dt;
-        x = x + ix; y = y + iy; z = z + iz;
-        p.disp.x[s] = p.disp.x[s] + ix;
-        p.disp.y[s] = p.disp.y[s] + iy;
-        p.disp.z[s] = p.disp.z[s] + iz;
-        if (S.periodic) {   // engine/boundary_condition.cpp:45-48
-            x = x - floor(x / S.L[0]) * S.L[0];
-            y = y - floor(y / S.L[1]) * S.L[1];
-            z = z - floor(z / S.L[2]) * S.L[2];
+    const bool active = s - S.atom_off < S.n;
+    int c = -1;
+    if (active) {
+        double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+        if (INTEGRATE) {
+            const double dt = S.dt;
+            // v += (1/2) F dt   (the stored force: zero before the first calculation)
+            double vx = p.vel.x[s] + ((1. / 2.) * p.force.x[s]) * dt;
+            double vy = p.vel.y[s] + ((1. / 2.) * p.force.y[s]) * dt;
+            double vz = p.vel.z[s] + ((1. / 2.) * p.force.z[s]) * dt;
+            // r += v dt ; d += v dt  (same increment)
+            double ix = vx * dt, iy = vy * dt, iz = vz * dt;
+            x = x + ix; y = y + iy; z = z + iz;
+            p.disp.x[s] = p.disp.x[s] + ix;
+            p.disp.y[s] = p.disp.y[s] + iy;
+            p.disp.z[s] = p.disp.z[s] + iz;
+            if (S.periodic) {   // engine/boundary_condition.cpp:45-48
+                x = x - floor(x / S.L[0]) * S.L[0];
+                y = y - floor(y / S.L[1]) * S.L[1];
+                z = z - floor(z / S.L[2]) * S.L[2];
+            }
+            p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+            p.pos.x[s] = x; p.pos.y[s] = y; p.pos.z[s] = z;
         }
-        p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
-        p.pos.x[s] = x; p.pos.y[s] = y; p.pos.z[s] = z;
+        if (KEYS) {
+            const int kx = cell_axis(x, S.q[0], S.nc[0]);
+            const int ky = cell_axis(y, S.q[1], S.nc[1]);
+            const int kz = cell_axis(z, S.q[2], S.nc[2]);
+            c = S.cell_off + (kx * S.nc[1] + ky) * S.nc[2] + kz;
+        }
     }
     if (KEYS) {
-        int kx = cell_axis(x, S.q[0], S.nc[0]);
-        int ky = cell_axis(y, S.q[1], S.nc[1]);
-        int kz = cell_axis(z, S.q[2], S.nc[2]);
-        int c = S.cell_off + (kx * S.nc[1] + ky) * S.nc[2] + kz;
-        p.cell_of[s] = c;
-        p.slot_of[s] = atomicAdd(&p.count[c], 1);
+        // Slots arrive nearly cell-sorted (the previous step's order), so equal
+        // keys form runs across the wave: one atomic per run reserves the run's
+        // arrival ranks, handed to its members by position.
+        const int lane = threadIdx.x & 63;
+        const int prev = __shfl_up(c, 1, 64);
+        const bool head = (lane == 0) || (c != prev);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long upto =
+            (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);   // bits 0..lane
+        const int h = 63 - __builtin_clzll(heads & upto);
+        const unsigned long long after = heads & ~upto;
+        const int end = after ? __builtin_ctzll(after) : 64;
+        int base = 0;
+        if (head && c >= 0) base = atomicAdd(&p.count[c], end - lane);
+        base = __shfl(base, h, 64);
+        if (c >= 0) {
+            p.cell_of[s] = c;
+            p.slot_of[s] = base + (lane - h);
+        }
     }
 }
 
@@ -355,26 +382,31 @@ __device__ __forceinline__ void for_each_neighbour_fp64(const SysDev& S, const S
     }
 }
 
+// FP64 pair law.  One IEEE division (1 / r^2); r^-6, the spline term and
+// F = W sep / r^2 are formed from it by multiplication, within a few ulp of
+// the reference's divide-per-term arithmetic.
 template <int LAW>
 __device__ __forceinline__ void pair_fp64(const SysDev& S, double dx, double dy, double dz,
                                           double r2, double& fx, double& fy, double& fz,
                                           double& pe, double& w)
 {
-    double vv, ww;
+    double vv, ww, fm;
     if (LAW == LAW_LJ) {
-        const double r2r = (S.sigma2 == 1.0) ? r2 : r2 / S.sigma2;
-        const double s6 = 1.0 / (r2r * r2r * r2r);
-        vv = 4.0 * s6 * (s6 - 1.0) + S.alpha + S.beta * (r2r / S.rc2_red - 1.0);
-        ww = 24.0 * s6 * (2.0 * s6 - 1.0) - 2.0 * S.beta * (r2r / S.rc2_red);
+        const double inv = 1.0 / r2;
+        const double t = (S.sigma2 == 1.0) ? inv : S.sigma2 * inv;   // (sigma / r)^2
+        const double s6 = t * t * t;
+        const double u = r2 * S.inv_rc2s;                            // (r / rc)^2
+        vv = 4.0 * s6 * (s6 - 1.0) + S.alpha + S.beta * (u - 1.0);
+        ww = 24.0 * s6 * (2.0 * s6 - 1.0) - 2.0 * S.beta * u;
         if (S.eps != 1.0) { vv = S.eps * vv; ww = S.eps * ww; }
-        fx += ww * dx / r2; fy += ww * dy / r2; fz += ww * dz / r2;
+        fm = ww * inv;
     } else {   // constant-magnitude test law
         const double nrm = sqrt(r2);
         vv = S.strength * (S.rc - nrm);
         ww = S.strength * nrm;
-        const double n2 = nrm * nrm;
-        fx += ww * dx / n2; fy += ww * dy / n2; fz += ww * dz / n2;
+        fm = ww / (nrm * nrm);
     }
+    fx += fm * dx; fy += fm * dy; fz += fm * dz;
     pe += vv;
     w += ww;
 }
@@ -387,6 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     const int s = b * kBlock + threadIdx.x;
     const bool active = (s - S.atom_off) < S.n;
     double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
+    int nct = 0;
     if (active) {
         if (LAW == LAW_NONE) {
             fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s];
@@ -394,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
             for_each_neighbour_fp64(S, p, s, [&](int, double dx, double dy, double dz, double r2,
                                                  int, int, int) {
                 pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+                ++nct;
             });
         } else {
             // cell-local float coordinates; sep = (l_i - l_j) + shift, where
@@ -428,6 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
                             const float dz = (li.z - lj.z) + shz;
                             const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                             if (r2 < rc2f) {
+                                ++nct;
                                 if (LAW == LAW_LJ) {
                                     const float inv = __builtin_amdgcn_rcpf(r2);
                                     const float t = inv * inv_s2;
@@ -466,6 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     }
     double acc[4] = {0, 0, 0, 0};
     if (active) {
+        if (p.ncount) p.ncount[s] = nct;
         if (LAW != LAW_NONE) { p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz; }
         double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
         if (KICK) {
@@ -489,6 +525,299 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// 6b. tiled pair kernel: one wave64 per home cell.
+//
+//   * the home cell's n_i particles (chunks of <= 64) are spread over the wave
+//     as S = 64 / n_i "slices" of n_i lanes; lane (i, slice) owns particle i
+//     and every S-th staged neighbour starting at its slice;
+//   * the 27 neighbour cells (reference lattice-image rule) are staged in LDS
+//     in coalesced reads, pre-shifted by their image (mixed) or with the
+//     image shift kept beside the coordinates (FP64, so that r^2 is formed
+//     exactly as the reference forms it);
+//   * phase 1 tests at most 64 staged candidates per lane and records the
+//     in-cutoff ones in a 64-bit mask held in registers; phase 2 walks the set
+//     bits, so the expensive pair law runs only for in-cutoff pairs and the
+//     wave's divergence is bounded by the busiest lane;
+//   * slice partial forces are summed in a fixed order through LDS; the
+//     owning lane writes F, applies the second half kick and accumulates
+//     |v|^2 and |d|^2, and the block sums go to the cell-space partials.
+// ---------------------------------------------------------------------------
+constexpr int kStageMixed = 512;   // staged neighbours per wave (float4 each)
+constexpr int kStageFp64 = 256;    // staged neighbours per wave (x,sx,y,sy,z,sz)
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int LAW>
+__device__ __forceinline__ void pair_mixed(const SysDev& S, float dx, float dy, float dz, float r2,
+                                           float inv_s2, float ur, float al, float be,
+                                           float& ax, float& ay, float& az, double& pe, double& w)
+{
+    if (LAW == LAW_LJ) {
+        const float inv = __builtin_amdgcn_rcpf(r2);
+        const float t = inv * inv_s2;
+        const float s6 = t * t * t;
+        const float u = r2 * ur;
+        const float vv = __builtin_fmaf(4.0f * s6, s6 - 1.0f, __builtin_fmaf(be, u - 1.0f, al));
+        const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f, -2.0f * be * u);
+        const float fm = ww * inv;
+        ax = __builtin_fmaf(fm, dx, ax);
+        ay = __builtin_fmaf(fm, dy, ay);
+        az = __builtin_fmaf(fm, dz, az);
+        pe += (double)vv;
+        w += (double)ww;
+    } else {
+        const float nrm = sqrtf(r2);
+        const float vv = (float)S.strength * ((float)S.rc - nrm);
+        const float ww = (float)S.strength * nrm;
+        const float fm = ww / r2;
+        ax += fm * dx; ay += fm * dy; az += fm * dz;
+        pe += (double)vv;
+        w += (double)ww;
+    }
+}
+
+template <int PREC, int LAW, bool KICK>
+__global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
+{
+    constexpr int CAP = (PREC == MIXED) ? kStageMixed : kStageFp64;
+    constexpr int PAD = CAP + 64;    // room for the dummy tail read by the last round
+    constexpr int NW = kBlock / 64;
+    // per-wave staging buffers
+    __shared__ float4 st_f[(PREC == MIXED) ? NW * PAD : 1];
+    __shared__ double2 st_x[(PREC == FP64) ? NW * PAD : 1];
+    __shared__ double2 st_y[(PREC == FP64) ? NW * PAD : 1];
+    __shared__ double2 st_z[(PREC == FP64) ? NW * PAD : 1];
+    __shared__ double red[NW][64][4];
+    __shared__ int tb_start[NW][27], tb_off[NW][27];
+    __shared__ float4 tb_shf[(PREC == MIXED) ? NW : 1][27];
+    __shared__ double tb_shd[(PREC == FP64) ? NW : 1][27][3];
+
+    const int b = blockIdx.x;
+    const int k = p.cblk_sys[b];
+    const SysDev& S = p.sys[k];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
+    double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
+
+    if (lc < S.ncell) {
+        const int c = S.cell_off + lc;
+        int cx, cy, cz;
+        cell_coords(lc, S.nc, cx, cy, cz);
+        const int h0 = p.start[c], h1 = p.start[c + 1];
+        float4* sf = st_f + ((PREC == MIXED) ? wid * PAD : 0);
+        double2* sx = st_x + ((PREC == FP64) ? wid * PAD : 0);
+        double2* sy = st_y + ((PREC == FP64) ? wid * PAD : 0);
+        double2* sz = st_z + ((PREC == FP64) ? wid * PAD : 0);
+        // r^2 < rc^2 and r^2 != 0 (the particle itself) in one unsigned compare
+        const unsigned rc2m1 = __float_as_uint((float)S.rc2) - 1u;
+        const double rc2d = S.rc2;
+        const float inv_s2 = (float)(1.0 / S.sigma2);
+        const float ur = (float)(1.0 / (S.sigma2 * S.rc2_red));
+        const float al = (float)S.alpha, be = (float)S.beta;
+
+        // ---- the 27 neighbour cells (reference lattice-image rule), one lane
+        //      each: slot range, image shift, and offsets in the concatenation
+        int cntq = 0;
+        if (lane < 27) {
+            const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+            int nbx, ix, nby, iy, nbz, iz;
+            nb_axis(cx, ox, S.nc[0], nbx, ix);
+            nb_axis(cy, oy, S.nc[1], nby, iy);
+            nb_axis(cz, oz, S.nc[2], nbz, iz);
+            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+            const int j0 = p.start[nb];
+            cntq = p.start[nb + 1] - j0;
+            tb_start[wid][lane] = j0;
+            if (PREC == MIXED)
+                tb_shf[wid][lane] = make_float4(
+                    (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]),
+                    (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]),
+                    (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]), 0.f);
+            else {
+                tb_shd[wid][lane][0] = ix * S.L[0];
+                tb_shd[wid][lane][1] = iy * S.L[1];
+                tb_shd[wid][lane][2] = iz * S.L[2];
+            }
+        }
+        const int incl = wave_incl_scan(cntq);
+        if (lane < 27) tb_off[wid][lane] = incl - cntq;
+        const int total = __shfl(incl, 26, 64);
+        wave_lds_sync();
+
+        for (int i0 = h0; i0 < h1; i0 += 64) {
+            const int ni = min(64, h1 - i0);
+            const int nsl = 64 / ni;
+            const bool act = lane < nsl * ni;
+            const int il = act ? lane % ni : 0;
+            const int sl = act ? lane / ni : 0;
+            const int si = i0 + il;
+            // the lane's own particle
+            double xi = 0, yi = 0, zi = 0;
+            float4 li = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (PREC == FP64) { xi = p.pos.x[si]; yi = p.pos.y[si]; zi = p.pos.z[si]; }
+            else li = p.lpos[si];
+            double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            int nct = 0;
+            const int cap = min(CAP, 64 * nsl);
+
+            for (int base = 0; base < total; base += cap) {
+                // ---- stage neighbours [base, base + m) of the concatenated 27 cells
+                const int m = min(cap, total - base);
+                for (int t = lane; t < m; t += 64) {
+                    const int g = base + t;
+                    int lo_q = 0, hi_q = 26;       // last cell q with off[q] <= g
+                    while (lo_q < hi_q) {
+                        const int mid = (lo_q + hi_q + 1) >> 1;
+                        if (tb_off[wid][mid] <= g) lo_q = mid; else hi_q = mid - 1;
+                    }
+                    const int j = tb_start[wid][lo_q] + (g - tb_off[wid][lo_q]);
+                    if (PREC == MIXED) {
+                        const float4 lj = p.lpos[j];
+                        const float4 sh = tb_shf[wid][lo_q];
+                        sf[t] = make_float4(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z, 0.f);
+                    } else {
+                        sx[t] = make_double2(p.pos.x[j], tb_shd[wid][lo_q][0]);
+                        sy[t] = make_double2(p.pos.y[j], tb_shd[wid][lo_q][1]);
+                        sz[t] = make_double2(p.pos.z[j], tb_shd[wid][lo_q][2]);
+                    }
+                }
+                // dummy tail: far away, never inside the cutoff
+                if (lane < nsl) {
+                    if (PREC == MIXED) sf[m + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+                    else {
+                        sx[m + lane] = make_double2(1e150, 0.0);
+                        sy[m + lane] = make_double2(1e150, 0.0);
+                        sz[m + lane] = make_double2(1e150, 0.0);
+                    }
+                }
+                wave_lds_sync();
+                // ---- phase 1: cutoff test of <= 64 candidates per lane (uniform trip count)
+                const int kmax = (m + nsl - 1) / nsl;
+                unsigned lo = 0u, hi = 0u;
+                auto test = [&](int kb) -> bool {
+                    const int q = sl + kb * nsl;
+                    if (PREC == MIXED) {
+                        const float4 d = sf[q];
+                        const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        return (__float_as_uint(r2) - 1u) < rc2m1;
+                    } else {
+                        const double2 X = sx[q], Y = sy[q], Z = sz[q];
+                        const double dx = (xi - X.x) - X.y;
+                        const double dy = (yi - Y.x) - Y.y;
+                        const double dz = (zi - Z.x) - Z.y;
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        return (r2 < rc2d) & (r2 > 0.0);
+                    }
+                };
+                const int k1 = min(kmax, 32);
+#pragma unroll 4
+                for (int kb = 0; kb < k1; ++kb) lo |= (test(kb) ? 1u : 0u) << kb;
+#pragma unroll 4
+                for (int kb = 32; kb < kmax; ++kb) hi |= (test(kb) ? 1u : 0u) << (kb - 32);
+                unsigned long long mask = act ? (((unsigned long long)hi << 32) | lo) : 0ull;
+                nct += __popcll(mask);
+                // ---- phase 2: pair law on the in-cutoff candidates only
+                float spe = 0.f, sw = 0.f;
+                while (mask) {
+                    const int kb = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const int q = sl + kb * nsl;
+                    if (PREC == MIXED) {
+                        const float4 d = sf[q];
+                        const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        if (LAW == LAW_LJ) {
+                            const float inv = __builtin_amdgcn_rcpf(r2);
+                            const float t = inv * inv_s2;
+                            const float s6 = t * t * t;
+                            const float u = r2 * ur;
+                            spe += __builtin_fmaf(4.0f * s6, s6 - 1.0f, __builtin_fmaf(be, u - 1.0f, al));
+                            const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f, -2.0f * be * u);
+                            sw += ww;
+                            const float fm = ww * inv;
+                            ax = __builtin_fmaf(fm, dx, ax);
+                            ay = __builtin_fmaf(fm, dy, ay);
+                            az = __builtin_fmaf(fm, dz, az);
+                        } else {
+                            const float nrm = sqrtf(r2);
+                            const float ww = (float)S.strength * nrm;
+                            spe += (float)S.strength * ((float)S.rc - nrm);
+                            sw += ww;
+                            const float fm = ww / r2;
+                            ax += fm * dx; ay += fm * dy; az += fm * dz;
+                        }
+                    } else {
+                        const double2 X = sx[q], Y = sy[q], Z = sz[q];
+                        const double dx = (xi - X.x) - X.y;
+                        const double dy = (yi - Y.x) - Y.y;
+                        const double dz = (zi - Z.x) - Z.y;
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+                    }
+                }
+                // per-stage float partials (<= 64 pairs) are folded into double sums
+                pe += (double)spe;
+                w += (double)sw;
+                wave_lds_sync();
+            }
+            if (PREC == MIXED) {
+                if (LAW == LAW_LJ && S.eps != 1.0) {
+                    ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
+                    pe *= S.eps; w *= S.eps;
+                }
+                fx = ax; fy = ay; fz = az;
+            }
+            // ---- sum the slices of each particle in a fixed order
+            red[wid][lane][0] = fx;
+            red[wid][lane][1] = fy;
+            red[wid][lane][2] = fz;
+            red[wid][lane][3] = (double)nct;
+            wave_lds_sync();
+            if (act && sl == 0) {
+                for (int t = 1; t < nsl; ++t) {
+                    fx += red[wid][il + t * ni][0];
+                    fy += red[wid][il + t * ni][1];
+                    fz += red[wid][il + t * ni][2];
+                    nct += (int)red[wid][il + t * ni][3];
+                }
+                if (p.ncount) p.ncount[si] = nct;
+                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
+                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+                if (KICK) {
+                    const double dt = S.dt;
+                    vx = vx + ((1. / 2.) * fx) * dt;
+                    vy = vy + ((1. / 2.) * fy) * dt;
+                    vz = vz + ((1. / 2.) * fz) * dt;
+                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
+                }
+                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                wacc[0] += (vx * vx + vy * vy) + vz * vz;
+                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
+            }
+            if (act) {
+                wacc[1] += (1. / 2.) * pe;
+                wacc[2] += (1. / 2.) * w;
+            }
+            wave_lds_sync();
+        }
+    }
+    double out[4];
+    block_sum4(wacc, out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 7. per-system reduction of the block sums into a thermodynamics record
 //    (physics/measurements.hpp:59-70; time += dt as engine/integrator.cpp:85)
@@ -506,6 +835,7 @@ struct FinalArgs {
     int nsys;
     int cap;
     int mode;
+    int cell_blocks;   // partial sums come from cell-space blocks (tiled pair kernel)
 };
 
 __global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
@@ -513,8 +843,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
     const int k = blockIdx.x;
     const SysDev& S = f.sys[k];
     double acc[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < S.nblk; b += kBlock)
-        for (int q = 0; q < 4; ++q) acc[q] += f.partial[4 * (S.blk_first + b) + q];
+    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
+    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+        for (int q = 0; q < 4; ++q) acc[q] += f.partial[4 * (b0 + b) + q];
     double out[4];
     block_sum4(acc, out);
     if (threadIdx.x != 0) return;

@@ -19,6 +19,7 @@ from lennardjonesium._native import check
 
 LAWS = {"lj": 0, "constant": 1, "none": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
+PAIR_KERNELS = {"particle": 0, "tiled": 1}
 THERMO_FIELDS = ("time", "kinetic_energy", "potential_energy", "total_energy", "virial",
                  "temperature", "mean_square_displacement")
 
@@ -67,7 +68,7 @@ def _arr(a, n):
 class Engine:
     """GPU engine for ``systems`` (list of SystemSpec), precision 'fp64' or 'mixed'."""
 
-    def __init__(self, systems, precision="fp64", graphs=True):
+    def __init__(self, systems, precision="fp64", graphs=True, pair_kernel="tiled"):
         self.lib = _native.engine()
         self.specs = list(systems)
         self.precision = precision
@@ -78,6 +79,8 @@ class Engine:
         self.h = h
         if not graphs:
             check(self.lib.lj_engine_set_graphs(self.h, 0))
+        if pair_kernel != "tiled":
+            self.set_pair_kernel(pair_kernel)
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -178,6 +181,12 @@ class Engine:
                 return idx[:cnt.value], sep[:cnt.value]
             cap = cnt.value
 
+    def neighbor_counts(self, sys):
+        n = self.specs[sys].particle_count
+        cnt = np.zeros(n, dtype=np.int32)
+        check(self.lib.lj_engine_neighbor_counts(self.h, sys, cnt.ctypes.data_as(_native.P_i)))
+        return cnt
+
     # -- observers ------------------------------------------------------------
     def rdf_sample(self, rmax=4.0, bins=200):
         check(self.lib.lj_engine_rdf_sample(self.h, float(rmax), int(bins)))
@@ -191,6 +200,10 @@ class Engine:
     # -- tuning / timing ---------------------------------------------------------
     def set_graphs(self, enabled):
         check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
+
+    def set_pair_kernel(self, variant):
+        """'tiled' (one wave per home cell, LDS-staged neighbours) or 'particle'."""
+        check(self.lib.lj_engine_set_pair_kernel(self.h, PAIR_KERNELS[variant]))
 
     def set_profiling(self, enabled):
         check(self.lib.lj_engine_set_profiling(self.h, 1 if enabled else 0))

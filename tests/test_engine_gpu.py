@@ -45,8 +45,8 @@ def force_err(F, Fref):
     return (np.linalg.norm(F - Fref, axis=1) / den).max()
 
 
-def make(eng, n, L, precision="fp64", **kw):
-    e = eng.Engine([eng.SystemSpec(n, L, **kw)], precision=precision)
+def make(eng, n, L, precision="fp64", pair_kernel="tiled", **kw):
+    e = eng.Engine([eng.SystemSpec(n, L, **kw)], precision=precision, pair_kernel=pair_kernel)
     return e
 
 
@@ -104,15 +104,53 @@ def test_pair_set_bit_exact(n, L, rc):
     e.close()
 
 
+def _oracle_counts(pos, L, rc=2.5):
+    idx, _ = ljref.pairs_cell(pos, L, rc)
+    n = len(pos)
+    return np.bincount(idx[:, 0], minlength=n) + np.bincount(idx[:, 1], minlength=n)
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+@pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.3), (500, 1.2, 0.4), (40, 0.05, 0.5),
+                                       (4000, 0.3, 1.0), (256000, 0.8442, 0.2)])
+def test_production_kernel_cutoff_decisions_bit_exact(n, rho, amp, kernel):
+    """Per-particle in-cutoff neighbour counts of the production pair kernels
+    (FP64) equal the reference filter's, particle by particle."""
+    eng = _engine_mod()
+    _say("neighbour counts", n, rho, kernel)
+    pos, L = perturbed_lattice(n, rho, amp, 17)
+    e = make(eng, n, L, pair_kernel=kernel)
+    e.set_state(0, pos, np.zeros_like(pos))
+    got = e.neighbor_counts(0)
+    assert np.array_equal(got, _oracle_counts(pos, L))
+    e.close()
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_mixed_kernel_cutoff_decisions(kernel):
+    """FP32 separations may flip only pairs within ~1e-5 of the cutoff."""
+    eng = _engine_mod()
+    pos, L = perturbed_lattice(32000, 0.8442, 0.2, 19)
+    e = make(eng, 32000, L, precision="mixed", pair_kernel=kernel)
+    e.set_state(0, pos, np.zeros_like(pos))
+    got = e.neighbor_counts(0)
+    ref = _oracle_counts(pos, L)
+    idx, sep = ljref.pairs_cell(pos, L, 2.5)
+    near = (np.abs(np.sqrt((sep ** 2).sum(1)) - 2.5) < 1e-5).sum()
+    assert np.abs(got - ref).sum() <= 2 * near + 4
+    e.close()
+
+
 # ---- forces, PE, virial: FP64 within 1e-12 ------------------------------------------
 
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
 @pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.2), (4000, 0.8, 0.3), (500, 1.2, 0.2),
                                        (256000, 0.8442, 0.15)])
-def test_forces_fp64(n, rho, amp):
+def test_forces_fp64(n, rho, amp, kernel):
     eng = _engine_mod()
-    _say("forces", n)
+    _say("forces", n, kernel)
     pos, L = perturbed_lattice(n, rho, amp, 11)
-    e = make(eng, n, L)
+    e = make(eng, n, L, pair_kernel=kernel)
     e.set_state(0, pos, np.zeros_like(pos))
     pe, w = e.compute_forces()
     st = e.get_state(0)
@@ -221,7 +259,8 @@ def test_trajectory_tracks_oracle_and_graphs_are_exact():
     assert rel[:100].max() < 1e-11 and rel.max() < 1e-6
 
 
-def test_batched_replicas_match_single_runs():
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_batched_replicas_match_single_runs(kernel):
     eng = _engine_mod()
     cases = [(500, 0.8, 0.7, 3), (256, 0.3, 1.5, 4), (864, 1.1, 0.9, 5)]
     specs, states = [], []
@@ -229,12 +268,12 @@ def test_batched_replicas_match_single_runs():
         st, L = ljref.initial_condition(n, rho, T, seed=seed)
         specs.append(eng.SystemSpec(n, L))
         states.append((st, L))
-    e = eng.Engine(specs)
+    e = eng.Engine(specs, pair_kernel=kernel)
     for k, (st, _) in enumerate(states):
         e.set_state(k, st.pos, st.vel)
     rec = e.step(100)
     for k, (st, L) in enumerate(states):
-        single = eng.Engine([specs[k]])
+        single = eng.Engine([specs[k]], pair_kernel=kernel)
         single.set_state(0, st.pos, st.vel)
         r1 = single.step(100)[:, 0, :]
         assert np.array_equal(rec[:, k, :], r1)      # batching does not change the arithmetic
@@ -258,11 +297,12 @@ def test_set_temperature_and_measure():
 
 # ---- mixed precision: error stated against the FP64 oracle -------------------------------
 
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
 @pytest.mark.parametrize("n,rho", [(4000, 0.8), (256000, 0.8442)])
-def test_mixed_forces_error(n, rho):
+def test_mixed_forces_error(n, rho, kernel):
     eng = _engine_mod()
     pos, L = perturbed_lattice(n, rho, 0.15, 3)
-    e = make(eng, n, L, precision="mixed")
+    e = make(eng, n, L, precision="mixed", pair_kernel=kernel)
     e.set_state(0, pos, np.zeros_like(pos))
     pe, w = e.compute_forces()
     F = e.get_state(0)["force"]
