@@ -134,6 +134,12 @@ int lj_engine_set_temperature(lj_engine* e, int sys, double target, double* meas
  * for every system, without stepping; out receives nsys rows. */
 int lj_engine_measure(lj_engine* e, lj_thermo* out);
 
+/* Freezes (active = 0) or resumes system `sys`: a frozen system is not
+ * integrated, produces no records and keeps its time, so replicas of a
+ * batch that have finished their schedule stop consuming steps (the role of
+ * a finished worker in api::SimulationPool, api/simulation_pool.cpp:93-101). */
+int lj_engine_set_active(lj_engine* e, int sys, int active);
+
 /* ---- parity hooks ---------------------------------------------------- */
 
 /* Integer cell coordinates of every particle of system `sys` on the current

@@ -37,6 +37,67 @@ double lj_initial_condition(int n, double density, double temperature, unsigned 
 int lj_format_double(double value, char* buf, int cap);
 
 
+/* api::Simulation (api/simulation.hpp:81-161) over the GPU engine.  The
+ * configuration mirrors api::Configuration (api/configuration.hpp:35-102);
+ * NULL strings take the reference's defaults. */
+typedef struct lj_sim_config {
+    double temperature;
+    double density;
+    int particle_count;
+    unsigned int random_seed;
+    double cutoff_distance;
+    double time_delta;
+    /* equilibration phase (control/simulation_phase.hpp:120-160) */
+    const char* equilibration_name;
+    double equilibration_tolerance;
+    int equilibration_sample_size;
+    int adjustment_interval;
+    int steady_state_time;
+    int equilibration_timeout;
+    /* observation phase (control/simulation_phase.hpp:200-244) */
+    const char* observation_name;
+    double observation_tolerance;
+    int observation_sample_size;
+    int observation_interval;
+    int observation_count;
+    /* output files */
+    const char* event_log;
+    const char* thermodynamic_log;
+    const char* observation_log;
+    const char* snapshot_log;
+    /* engine precision: LJ_PRECISION_* of lj_engine.h */
+    int precision;
+} lj_sim_config;
+
+typedef struct lj_simulation lj_simulation;
+
+/* api::make_simulation (api/configuration.cpp:40-92): builds the initial
+ * condition and the force parameters; the GPU is not touched until run. */
+int lj_simulation_create(const lj_sim_config* cfg, lj_simulation** out);
+int lj_simulation_destroy(lj_simulation* sim);
+
+/* api::Simulation::run (api/simulation.cpp:72-117): equilibration then
+ * observation, writing the four output files; echo != 0 also prints each
+ * events.log line to stdout.  Re-runnable; files are overwritten. */
+int lj_simulation_run(lj_simulation* sim, int echo);
+
+/* Runs nsim simulations as one batched GPU engine (one launch per kernel for
+ * all replicas), each with its own schedule and files -- the role of
+ * api::SimulationPool (api/simulation_pool.cpp:35-101).  Replicas that finish
+ * are frozen.  status (optional, nsim ints) reads 3 while running, then
+ * 0 = completed, 1 = aborted in equilibration, 2 = aborted in observation. */
+int lj_simulation_run_batch(lj_simulation** sims, int nsim, int echo, int* status);
+
+/* physics::ShortRangeForce::potential / virial / force along z
+ * (physics/forces.hpp:80-94), exposed for plots as api::Simulation does
+ * (api/simulation.hpp:148-150). */
+double lj_simulation_potential(const lj_simulation* sim, double distance);
+double lj_simulation_virial(const lj_simulation* sim, double distance);
+double lj_simulation_force(const lj_simulation* sim, double distance);
+
+/* Message of the most recent failing host-layer call on this thread. */
+const char* lj_host_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,7 @@ COMMON = [
 ]
 
 
-HOST_SOURCES = ["initial_condition.cpp", "format.cpp"]
+HOST_SOURCES = ["initial_condition.cpp", "format.cpp", "simulation.cpp"]
 
 
 def _run(cmd, verbose, what):

@@ -411,6 +411,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             return fail(LJ_E_INVALID, "a force calculation needs the periodic box");
         }
         s.n = d.particle_count;
+        s.active = 1;
         s.law = d.force_law;
         s.periodic = d.periodic ? 1 : 0;
         s.dt = d.time_delta;
@@ -884,6 +885,16 @@ int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples)
     HIP_TRY(hipFree(e->d_rdf));
     e->d_rdf = nullptr;
     e->rdf_samples = 0;
+    return LJ_OK;
+}
+
+int lj_engine_set_active(lj_engine* e, int sys, int active)
+{
+    int rc = check_sys(e, sys);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->sys[sys].active = active ? 1 : 0;
+    HIP_TRY(hipMemcpy(e->d_sys + sys, &e->sys[sys], sizeof(SysDev), hipMemcpyHostToDevice));
     return LJ_OK;
 }
 

@@ -48,6 +48,8 @@ struct SysDev {
     int nblk;          // blocks of this system
     int cblk_first;    // first cell-space block (kCellsPerBlock cells each)
     int ncblk;         // cell-space blocks of this system
+    int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
+    int pad_;
     double L[3];       // box
     double q[3];       // nc / L (the reference forms this quotient first)
     double a[3];       // L / nc (cell side, used for cell-local coordinates)
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
     int c = -1;
     if (active) {
         double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
-        if (INTEGRATE) {
+        if (INTEGRATE && S.active) {
             const double dt = S.dt;
             // v += (1/2) F dt   (the stored force: zero before the first calculation)
             double vx = p.vel.x[s] + ((1. / 2.) * p.force.x[s]) * dt;
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     const int b = blockIdx.x;
     const SysDev& S = p.sys[p.blk_sys[b]];
     const int s = b * kBlock + threadIdx.x;
-    const bool active = (s - S.atom_off) < S.n;
+    const bool active = (s - S.atom_off) < S.n && (!KICK || S.active);
     double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
     int nct = 0;
     if (active) {
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
     const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
-    if (lc < S.ncell) {
+    if (lc < S.ncell && (!KICK || S.active)) {
         const int c = S.cell_off + lc;
         int cx, cy, cz;
         cell_coords(lc, S.nc, cx, cy, cz);
@@ -850,6 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
     double out[4];
     block_sum4(acc, out);
     if (threadIdx.x != 0) return;
+    if (f.mode == 0 && !S.active) return;
     double pe = f.pe[k], w = f.vir[k];
     if (f.mode != 2 && S.law != LAW_NONE) {
         pe = out[1];

@@ -80,6 +80,7 @@ def engine():
         "lj_engine_rdf_sample": (c_i, [V, c_d, c_i]),
         "lj_engine_rdf_read": (c_i, [V, P_d, P_i]),
         "lj_engine_neighbor_counts": (c_i, [V, c_i, P_i]),
+        "lj_engine_set_active": (c_i, [V, c_i, c_i]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),

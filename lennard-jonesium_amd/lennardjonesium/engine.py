@@ -149,6 +149,9 @@ class Engine:
         check(self.lib.lj_engine_set_temperature(self.h, sys, float(target), ctypes.byref(m)))
         return m.value
 
+    def set_active(self, sys, active):
+        check(self.lib.lj_engine_set_active(self.h, sys, 1 if active else 0))
+
     # -- parity hooks ---------------------------------------------------------
     def cell_keys(self, sys):
         n = self.specs[sys].particle_count
