@@ -177,6 +177,21 @@ int lj_engine_neighbor_counts(lj_engine* e, int sys, int* counts);
 int lj_engine_rdf_sample(lj_engine* e, double rmax, int bins);
 int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples);
 
+/* ---- differentiation ---------------------------------------------------- */
+
+/* Hand-written adjoint of the FP64 velocity-Verlet trajectory of a
+ * single-system LJ engine (no reference counterpart).  Starting from the
+ * engine's current state, with forces F(r0) computed first, it integrates
+ * nsteps steps and returns
+ *   loss = (1/nsteps) sum_{t=1..nsteps} [PE_t + rho (T_t + W_t / (3N))]
+ * (rho = density) and its exact gradients with respect to epsilon, sigma
+ * (rc = cutoff * sigma moves with sigma), the initial positions and
+ * velocities (N x 3, particle-id order).  States are checkpointed every
+ * `every` steps and recomputed segment by segment in the reverse sweep.
+ * The engine is left holding the initial state. */
+int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, double* loss,
+                      double* grad_eps, double* grad_sigma, double* grad_pos, double* grad_vel);
+
 /* ---- tuning ----------------------------------------------------------- */
 
 /* 0 disables HIP graph capture of the step loop (launches each kernel). */

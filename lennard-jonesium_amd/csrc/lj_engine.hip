@@ -947,3 +947,144 @@ int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms
 }
 
 }  // extern "C"
+
+namespace {
+
+struct DevBuf {
+    double* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, double* loss,
+                                 double* grad_eps, double* grad_sigma, double* grad_pos,
+                                 double* grad_vel)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (e->nsys != 1 || e->precision != LJ_PRECISION_FP64 || e->sys[0].law != LJ_LAW_LJ)
+        return fail(LJ_E_INVALID, "the adjoint needs a single-system FP64 LJ engine");
+    if (nsteps < 1 || every < 1) return fail(LJ_E_INVALID, "nsteps and checkpoint interval must be >= 1");
+    const SysDev S = e->sys[0];
+    const int n = S.n, n3 = 3 * n;
+    const int nck = nsteps / every + 1;
+    const double dt = S.dt, c1 = 1.0 / nsteps;
+    const double cT = c1 * 2.0 * density / (3.0 * n);   // d/dv of c1 * rho * T
+    const double cTke = 2.0 * density / (3.0 * n);       // rho * T = cTke * KE
+    const double cW = density / (3.0 * n);
+    const double eps = S.eps, sig = std::sqrt(S.sigma2);
+    DevBuf ck_pos, ck_vel, seg_pos, seg_vel, lam_r, lam_v, pend, phi, jphi, gradw, frc, part, acc;
+    if ((rc = dev_alloc(&ck_pos.p, (size_t)nck * n3)) || (rc = dev_alloc(&ck_vel.p, (size_t)nck * n3)) ||
+        (rc = dev_alloc(&seg_pos.p, (size_t)(every + 1) * n3)) ||
+        (rc = dev_alloc(&seg_vel.p, (size_t)(every + 1) * n3)) || (rc = dev_alloc(&lam_r.p, n3)) ||
+        (rc = dev_alloc(&lam_v.p, n3)) || (rc = dev_alloc(&pend.p, n3)) || (rc = dev_alloc(&phi.p, n3)) ||
+        (rc = dev_alloc(&jphi.p, n3)) || (rc = dev_alloc(&gradw.p, n3)) || (rc = dev_alloc(&frc.p, n3)) ||
+        (rc = dev_alloc(&part.p, 5 * (size_t)S.nblk)) || (rc = dev_alloc(&acc.p, 2)))
+        return rc;
+    const dim3 gb(S.nblk), tb(kBlock), gv((n3 + kBlock - 1) / kBlock);
+    auto to_id = [&](double* pos, double* vel) {
+        const ljk::StepArgs p = e->args(e->cur);
+        hipLaunchKernelGGL(ljk::k_to_id, gb, tb, 0, e->stream, p, 0, n, pos, vel);
+    };
+    auto from_id = [&](const double* pos, const double* vel) {
+        const ljk::StepArgs p = e->args(e->cur);
+        hipLaunchKernelGGL(ljk::k_from_id, dim3((n + kBlock - 1) / kBlock), tb, 0, e->stream, p, 0,
+                           n, pos, vel);
+        e->sorted = false;
+    };
+    auto forces_now = [&]() -> int {
+        int r = enqueue_cells(e, false);
+        if (!r) r = enqueue_pair(e, false, false);
+        if (!r) r = enqueue_finalize(e, 1);
+        return r;
+    };
+    auto adj_pair = [&](const double* ph) {
+        ljk::AdjArgs a;
+        a.p = e->args(e->cur);
+        a.sys = 0;
+        a.n = n;
+        a.phi = ph;
+        a.jphi = jphi.p;
+        a.gradw = gradw.p;
+        a.force = frc.p;
+        a.part = part.p;
+        hipLaunchKernelGGL(ljk::k_adjoint_pair, gb, tb, 0, e->stream, a);
+    };
+    const bool saved_graphs = e->use_graphs;
+    // ---- forward: F(r0), then nsteps with checkpoints every `every` steps
+    if ((rc = forces_now())) return rc;
+    to_id(ck_pos.p, ck_vel.p);
+    std::vector<lj_thermo> rec(nsteps);
+    for (int t = 0; t < nsteps;) {
+        const int k = std::min(every, nsteps - t);
+        if ((rc = lj_engine_step(e, k, rec.data() + t))) return rc;
+        t += k;
+        if (t % every == 0) to_id(ck_pos.p + (size_t)(t / every) * n3, ck_vel.p + (size_t)(t / every) * n3);
+    }
+    double L = 0;
+    for (int t = 0; t < nsteps; ++t)
+        L += rec[t].potential_energy + cTke * rec[t].kinetic_energy + cW * rec[t].virial;
+    L *= c1;
+    // ---- reverse sweep over segments, recomputing each from its checkpoint
+    HIP_TRY(hipMemsetAsync(lam_r.p, 0, n3 * sizeof(double), e->stream));
+    HIP_TRY(hipMemsetAsync(lam_v.p, 0, n3 * sizeof(double), e->stream));
+    HIP_TRY(hipMemsetAsync(pend.p, 0, n3 * sizeof(double), e->stream));
+    HIP_TRY(hipMemsetAsync(acc.p, 0, 2 * sizeof(double), e->stream));
+    const int nseg = (nsteps + every - 1) / every;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int t0 = sg * every, t1 = std::min(t0 + every, nsteps);
+        from_id(ck_pos.p + (size_t)sg * n3, ck_vel.p + (size_t)sg * n3);
+        if ((rc = forces_now())) return rc;
+        to_id(seg_pos.p, seg_vel.p);
+        for (int k = 1; k <= t1 - t0; ++k) {
+            if ((rc = enqueue_step(e))) return rc;
+            to_id(seg_pos.p + (size_t)k * n3, seg_vel.p + (size_t)k * n3);
+        }
+        for (int k = t1 - t0; k >= 1; --k) {
+            const double* vt = seg_vel.p + (size_t)k * n3;
+            hipLaunchKernelGGL(ljk::k_adj_pre, gv, tb, 0, e->stream, n3, 0.5 * dt, cT, lam_v.p, vt,
+                               pend.p, phi.p);
+            from_id(seg_pos.p + (size_t)k * n3, vt);
+            if ((rc = enqueue_cells(e, false))) return rc;
+            adj_pair(phi.p);
+            hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, S.nblk, c1, cW, eps,
+                               sig, acc.p);
+            hipLaunchKernelGGL(ljk::k_adj_post, gv, tb, 0, e->stream, n3, dt, c1, cW, cT, frc.p,
+                               gradw.p, jphi.p, vt, lam_r.p, lam_v.p, pend.p);
+        }
+    }
+    // ---- t = 0: the first half kick used F(r0)
+    from_id(ck_pos.p, ck_vel.p);
+    if ((rc = enqueue_cells(e, false))) return rc;
+    adj_pair(pend.p);
+    hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, S.nblk, 0.0, 0.0, eps, sig,
+                       acc.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<double> lr(n3), jp(n3), lv(n3);
+    double hacc[2];
+    HIP_TRY(hipMemcpy(lr.data(), lam_r.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(jp.data(), jphi.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lv.data(), lam_v.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hacc, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    // leave the engine holding the initial state with F(r0)
+    from_id(ck_pos.p, ck_vel.p);
+    if ((rc = forces_now())) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->use_graphs = saved_graphs;
+    double zero = 0.0;
+    HIP_TRY(hipMemcpy(e->d_time, &zero, sizeof(double), hipMemcpyHostToDevice));
+    if (loss) *loss = L;
+    if (grad_eps) *grad_eps = hacc[0];
+    if (grad_sigma) *grad_sigma = hacc[1];
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            if (grad_pos) grad_pos[3 * (size_t)i + a] = lr[(size_t)a * n + i] + jp[(size_t)a * n + i];
+            if (grad_vel) grad_vel[3 * (size_t)i + a] = lv[(size_t)a * n + i];
+        }
+    return LJ_OK;
+}

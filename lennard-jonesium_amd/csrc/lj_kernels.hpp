@@ -999,4 +999,174 @@ __global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bin
         if (lh[t]) atomicAdd(&hist[(size_t)k * bins + t], (unsigned long long)lh[t]);
 }
 
+
+// ===========================================================================
+// Adjoint (reverse-mode) kernels for the FP64 LJ trajectory of one system.
+//
+// With s = r^2, x^2 = s / sigma^2, S6 = x^-6, u = s / (sigma^2 rc^2):
+//   pair energy  e(s) = eps [4 S6 (S6 - 1) + alpha + beta (u - 1)]
+//   pair virial  w(s) = eps [24 S6 (2 S6 - 1) - 2 beta u]
+//   force        f = g(s) sep,  g = w / s
+//   w'(s) = eps [-72 S6 (4 S6 - 1) / s - 2 beta / (sigma^2 rc^2)],  g' = (w' - g) / s
+// so the force Jacobian acting on a per-particle vector phi is
+//   (J phi)_i = sum_j g (phi_i - phi_j) + 2 g' (sep . (phi_i - phi_j)) sep,
+// grad_i W = sum_j 2 w' sep, and the parameter derivatives are
+//   d/d eps: e / eps, w / eps, f / eps;
+//   d/d sigma: de = w / sigma, dw = eps [144 S6 (4 S6 - 1) + 4 beta u] / sigma,
+//              df = (dw / s) sep.
+// Vectors indexed by particle id live in three planes of length n.
+// ===========================================================================
+struct AdjArgs {
+    StepArgs p;
+    int sys;
+    int n;
+    const double* phi;     // [3][n] id order: force adjoint of this time
+    double* jphi;          // [3][n] out: J phi
+    double* gradw;         // [3][n] out: grad W
+    double* force;         // [3][n] out: F
+    double* part;          // per block: sum df/deps.phi, sum df/dsig.phi, U, W, sum dw/dsig / 2
+};
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_pair(AdjArgs a)
+{
+    const StepArgs& p = a.p;
+    const SysDev& S = p.sys[a.sys];
+    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
+    const bool active = s - S.atom_off < S.n;
+    double acc[5] = {0, 0, 0, 0, 0};
+    if (active) {
+        const int n = a.n;
+        const int i = p.id[s];
+        const double pxi = a.phi[i], pyi = a.phi[n + i], pzi = a.phi[2 * n + i];
+        double jx = 0, jy = 0, jz = 0, wx = 0, wy = 0, wz = 0, fx = 0, fy = 0, fz = 0;
+        const double eps = S.eps, sig = sqrt(S.sigma2);
+        for_each_neighbour_fp64(S, p, s, [&](int js, double dx, double dy, double dz, double r2,
+                                             int, int, int) {
+            const int j = p.id[js];
+            const double inv = 1.0 / r2;
+            const double t = S.sigma2 * inv;
+            const double s6 = t * t * t;
+            const double u = r2 * S.inv_rc2s;
+            const double e = eps * (4.0 * s6 * (s6 - 1.0) + S.alpha + S.beta * (u - 1.0));
+            const double w = eps * (24.0 * s6 * (2.0 * s6 - 1.0) - 2.0 * S.beta * u);
+            const double g = w * inv;
+            const double wp = eps * (-72.0 * s6 * (4.0 * s6 - 1.0) * inv - 2.0 * S.beta * S.inv_rc2s);
+            const double gp = (wp - g) * inv;
+            const double ddx = pxi - a.phi[j], ddy = pyi - a.phi[n + j], ddz = pzi - a.phi[2 * n + j];
+            const double sd = dx * ddx + dy * ddy + dz * ddz;
+            jx += g * ddx + 2.0 * gp * sd * dx;
+            jy += g * ddy + 2.0 * gp * sd * dy;
+            jz += g * ddz + 2.0 * gp * sd * dz;
+            wx += 2.0 * wp * dx; wy += 2.0 * wp * dy; wz += 2.0 * wp * dz;
+            fx += g * dx; fy += g * dy; fz += g * dz;
+            const double sphi = dx * pxi + dy * pyi + dz * pzi;
+            const double dw_dsig = eps * (144.0 * s6 * (4.0 * s6 - 1.0) + 4.0 * S.beta * u) / sig;
+            acc[0] += (g / eps) * sphi;
+            acc[1] += dw_dsig * inv * sphi;
+            acc[2] += 0.5 * e;
+            acc[3] += 0.5 * w;
+            acc[4] += 0.5 * dw_dsig;
+        });
+        a.jphi[i] = jx; a.jphi[n + i] = jy; a.jphi[2 * n + i] = jz;
+        a.gradw[i] = wx; a.gradw[n + i] = wy; a.gradw[2 * n + i] = wz;
+        a.force[i] = fx; a.force[n + i] = fy; a.force[2 * n + i] = fz;
+    }
+    __shared__ double red[5][kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 5; ++k) {
+        double v = wave_sum(acc[k]);
+        if (lane == 0) red[k][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double v = 0;
+        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+        a.part[5 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+// Copies the state of one system from its slots to id order (pos, vel).
+__global__ __launch_bounds__(kBlock) void k_to_id(StepArgs p, int sysk, int n, double* pos,
+                                                  double* vel)
+{
+    const SysDev& S = p.sys[sysk];
+    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
+    if (s - S.atom_off >= S.n) return;
+    const int i = p.id[s];
+    pos[i] = p.pos.x[s]; pos[n + i] = p.pos.y[s]; pos[2 * n + i] = p.pos.z[s];
+    vel[i] = p.vel.x[s]; vel[n + i] = p.vel.y[s]; vel[2 * n + i] = p.vel.z[s];
+}
+
+// Loads one system from id-ordered arrays into its slots (slot atom_off + i
+// holds particle i); displacements and forces are cleared.
+__global__ __launch_bounds__(kBlock) void k_from_id(StepArgs p, int sysk, int n, const double* pos,
+                                                    const double* vel)
+{
+    const SysDev& S = p.sys[sysk];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.n) return;
+    const int s = S.atom_off + i;
+    p.pos.x[s] = pos[i]; p.pos.y[s] = pos[n + i]; p.pos.z[s] = pos[2 * n + i];
+    p.vel.x[s] = vel[i]; p.vel.y[s] = vel[n + i]; p.vel.z[s] = vel[2 * n + i];
+    p.disp.x[s] = 0; p.disp.y[s] = 0; p.disp.z[s] = 0;
+    p.force.x[s] = 0; p.force.y[s] = 0; p.force.z[s] = 0;
+    p.id[s] = i;
+}
+
+// Reverse velocity Verlet, before the pair kernel at time t:
+//   phi = (dt/2) (lam_v + cT v_t) + pend
+__global__ __launch_bounds__(kBlock) void k_adj_pre(int n3, double hdt, double cT, const double* lam_v,
+                                                    const double* v, const double* pend, double* phi)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    phi[k] = hdt * (lam_v[k] + cT * v[k]) + pend[k];
+}
+
+// ... and after it:
+//   g = lam_r + c1 (-F + cW gradW) + J phi        (dL/dr_t)
+//   h = lam_v + cT v_t + dt g                      (dL/d v_half)
+//   lam_v <- h, pend <- (dt/2) h, lam_r <- g
+__global__ __launch_bounds__(kBlock) void k_adj_post(int n3, double dt, double c1, double cW,
+                                                     double cT, const double* F, const double* gw,
+                                                     const double* jphi, const double* v,
+                                                     double* lam_r, double* lam_v, double* pend)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    const double g = lam_r[k] + c1 * (-F[k] + cW * gw[k]) + jphi[k];
+    const double h = lam_v[k] + cT * v[k] + dt * g;
+    lam_r[k] = g;
+    lam_v[k] = h;
+    pend[k] = (0.5 * dt) * h;
+}
+
+
+// Accumulates one adjoint step's parameter terms (fixed-order block sums):
+//   acc[0] += sum df/deps.phi + c1 (U / eps + cW W / eps)
+//   acc[1] += sum df/dsig.phi + c1 (W / sig + cW dW/dsig)
+__global__ __launch_bounds__(kBlock) void k_adj_acc(const double* part, int nb, double c1, double cW,
+                                                    double eps, double sig, double* acc)
+{
+    double v[4] = {0, 0, 0, 0};
+    double v4 = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        v[0] += part[5 * b]; v[1] += part[5 * b + 1]; v[2] += part[5 * b + 2];
+        v[3] += part[5 * b + 3]; v4 += part[5 * b + 4];
+    }
+    double out[4];
+    block_sum4(v, out);
+    __shared__ double s4[kBlock / 64];
+    const double t4 = wave_sum(v4);
+    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = t4;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dws = 0;
+        for (int w = 0; w < kBlock / 64; ++w) dws += s4[w];
+        const double U = out[2], W = out[3];
+        acc[0] += out[0] + c1 * (U / eps + cW * W / eps);
+        acc[1] += out[1] + c1 * (W / sig + cW * dws);
+    }
+}
+
 }  // namespace ljk

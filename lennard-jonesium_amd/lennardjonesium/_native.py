@@ -81,6 +81,7 @@ def engine():
         "lj_engine_rdf_read": (c_i, [V, P_d, P_i]),
         "lj_engine_neighbor_counts": (c_i, [V, c_i, P_i]),
         "lj_engine_set_active": (c_i, [V, c_i, c_i]),
+        "lj_engine_adjoint": (c_i, [V, c_i, c_i, c_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),

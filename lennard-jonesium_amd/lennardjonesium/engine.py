@@ -152,6 +152,19 @@ class Engine:
     def set_active(self, sys, active):
         check(self.lib.lj_engine_set_active(self.h, sys, 1 if active else 0))
 
+    def adjoint(self, nsteps, density, every=100):
+        """Loss (1/n) sum_t [PE_t + rho (T_t + W_t / 3N)] over nsteps FP64 steps
+        from the current state and its gradients: returns
+        (loss, d/deps, d/dsigma, d/dr0 (N, 3), d/dv0 (N, 3))."""
+        n = self.specs[0].particle_count
+        L, ge, gs = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        gr = np.zeros((n, 3))
+        gv = np.zeros((n, 3))
+        check(self.lib.lj_engine_adjoint(self.h, int(nsteps), int(every), float(density),
+                                         ctypes.byref(L), ctypes.byref(ge), ctypes.byref(gs),
+                                         _dptr(gr), _dptr(gv)))
+        return L.value, ge.value, gs.value, gr, gv
+
     # -- parity hooks ---------------------------------------------------------
     def cell_keys(self, sys):
         n = self.specs[sys].particle_count
