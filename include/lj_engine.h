@@ -205,6 +205,12 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
 #define LJ_PAIR_CELL_TILED 1
 int lj_engine_set_pair_kernel(lj_engine* e, int variant);
 
+/* Project autotuner: times `steps` steps of every pair-kernel variant on the
+ * engine's own systems (from the current state, which is saved and restored,
+ * so tuning leaves no trace), then keeps the fastest.  chosen / ms_per_step
+ * (optional) receive the winner and its time per step. */
+int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step);
+
 /* Mean duration in milliseconds of each step-loop kernel over the last
  * lj_engine_step call with profiling enabled (HIP events on the engine's
  * stream).  names/ms receive up to cap entries; returns the entry count. */

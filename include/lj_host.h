@@ -28,6 +28,13 @@ double lj_cubic_lattice(int n, double density, int lattice, double* pos);
 double lj_initial_condition(int n, double density, double temperature, unsigned int seed,
                             int lattice, double* pos, double* vel);
 
+/* Replica sizes for batched sweeps: count integers in [lo, hi] drawn from
+ * std::mt19937(seed) by rejection sampling (u < floor(2^32 / R) * R,
+ * value lo + u mod R).  No reference counterpart: the reference's sweeps use
+ * one particle count (orchestration/sweep_configuration.py:55).  Returns 0,
+ * or -1 on bad arguments. */
+int lj_sweep_sizes(unsigned int seed, int count, int lo, int hi, int* out);
+
 /* Formats a double the way the reference's fmt "{}" sinks print it
  * (output/sinks.cpp:100-204): shortest round-trip digits, fixed notation
  * when the decimal exponent of the leading digit is in [-4, 16), otherwise

@@ -139,4 +139,20 @@ double lj_initial_condition(int n, double density, double temperature, unsigned 
     return L;
 }
 
+int lj_sweep_sizes(unsigned int seed, int count, int lo, int hi, int* out)
+{
+    if (count < 0 || hi < lo || !out) return -1;
+    std::mt19937 gen(seed);
+    const unsigned long long range = (unsigned long long)(hi - lo) + 1ull;
+    const unsigned long long limit = (0x100000000ull / range) * range;   // accept u < limit
+    for (int k = 0; k < count; ++k) {
+        unsigned long long u;
+        do {
+            u = gen();
+        } while (u >= limit);
+        out[k] = lo + (int)(u % range);
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -52,6 +52,14 @@ int dev_alloc(T** p, size_t count)
     return LJ_OK;
 }
 
+struct DevBuf {
+    double* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
 struct KernelTimer {
     const char* name;
     double total_ms = 0;
@@ -80,6 +88,7 @@ struct lj_engine {
     int* d_id[2] = {nullptr, nullptr};
     double* d_force = nullptr;
     int *d_cell_of = nullptr, *d_slot_of = nullptr, *d_perm = nullptr, *d_scell = nullptr;
+    int* d_perm_id = nullptr;
     float4* d_lpos = nullptr;
     int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
     double* d_partial = nullptr;
@@ -129,6 +138,7 @@ struct lj_engine {
         p.start = d_start;
         p.tile_sum = d_tile;
         p.perm = d_perm;
+        p.perm_id = d_perm_id;
         p.scell = d_scell;
         p.lpos = d_lpos;
         p.partial = d_partial;
@@ -506,7 +516,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     }
     if (e->has_cells) {
         if ((rc = dev_alloc(&e->d_cell_of, ns)) || (rc = dev_alloc(&e->d_slot_of, ns)) ||
-            (rc = dev_alloc(&e->d_perm, ns)) || (rc = dev_alloc(&e->d_scell, ns)) ||
+            (rc = dev_alloc(&e->d_perm, ns)) || (rc = dev_alloc(&e->d_perm_id, ns)) ||
+            (rc = dev_alloc(&e->d_scell, ns)) ||
             (rc = dev_alloc(&e->d_count, e->ncell_total)) ||
             (rc = dev_alloc(&e->d_count_init, e->ncell_total)) ||
             (rc = dev_alloc(&e->d_start, e->ncell_total + 1)) ||
@@ -553,6 +564,7 @@ int lj_engine_destroy(lj_engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
+                    e->d_perm_id,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_start, e->d_tile,
                     e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_records,
                     e->d_measure, e->d_rdf};
@@ -920,6 +932,79 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
     return LJ_OK;
 }
 
+int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->has_cells) {
+        if (chosen) *chosen = e->pair_variant;
+        return LJ_OK;
+    }
+    if (steps < 2) steps = 2;
+    // save the state (current buffer set, forces, clocks) so trials leave no trace
+    const size_t ns = e->nslot;
+    DevBuf sp, sv, sd, sf, st;
+    int* sid = nullptr;
+    if ((rc = dev_alloc(&sp.p, 3 * ns)) || (rc = dev_alloc(&sv.p, 3 * ns)) ||
+        (rc = dev_alloc(&sd.p, 3 * ns)) || (rc = dev_alloc(&sf.p, 3 * ns)) ||
+        (rc = dev_alloc(&st.p, 3 * (size_t)e->nsys)) || (rc = dev_alloc(&sid, ns)))
+        return rc;
+    const int cur0 = e->cur;
+    const bool sorted0 = e->sorted;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e->stream);
+    };
+    auto save = [&]() {
+        hipError_t h = cp(sp.p, e->d_pos[e->cur], 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(sv.p, e->d_vel[e->cur], 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(sd.p, e->d_disp[e->cur], 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(sf.p, e->d_force, 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(sid, e->d_id[e->cur], ns * sizeof(int));
+        if (h == hipSuccess) h = cp(st.p, e->d_time, e->nsys * sizeof(double));
+        if (h == hipSuccess) h = cp(st.p + e->nsys, e->d_pe, e->nsys * sizeof(double));
+        if (h == hipSuccess) h = cp(st.p + 2 * e->nsys, e->d_vir, e->nsys * sizeof(double));
+        return h;
+    };
+    auto restore = [&]() {
+        e->cur = cur0;
+        e->sorted = sorted0;
+        hipError_t h = cp(e->d_pos[cur0], sp.p, 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_vel[cur0], sv.p, 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_disp[cur0], sd.p, 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_force, sf.p, 3 * ns * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_id[cur0], sid, ns * sizeof(int));
+        if (h == hipSuccess) h = cp(e->d_time, st.p, e->nsys * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_pe, st.p + e->nsys, e->nsys * sizeof(double));
+        if (h == hipSuccess) h = cp(e->d_vir, st.p + 2 * e->nsys, e->nsys * sizeof(double));
+        return h;
+    };
+    hipError_t he = save();
+    if (he != hipSuccess) { (void)hipFree(sid); return fail(LJ_E_HIP, "autotune: save failed"); }
+    const int variants[2] = {LJ_PAIR_CELL_TILED, LJ_PAIR_PER_PARTICLE};
+    double best = 1e300;
+    int best_v = e->pair_variant;
+    for (int v : variants) {
+        if ((rc = lj_engine_set_pair_kernel(e, v))) break;
+        if ((rc = lj_engine_step(e, 2, nullptr))) break;   // warm-up (and graph build)
+        HIP_TRY(hipEventRecord(e->ev0, e->stream));
+        if ((rc = lj_engine_step(e, steps, nullptr))) break;
+        HIP_TRY(hipEventRecord(e->ev1, e->stream));
+        HIP_TRY(hipEventSynchronize(e->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        if (ms < best) { best = ms; best_v = v; }
+        if ((he = restore()) != hipSuccess) break;
+    }
+    (void)hipFree(sid);
+    if (he != hipSuccess) return fail(LJ_E_HIP, "autotune: restore failed");
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if ((rc = lj_engine_set_pair_kernel(e, best_v))) return rc;
+    if (chosen) *chosen = best_v;
+    if (ms_per_step) *ms_per_step = best / steps;
+    return LJ_OK;
+}
+
 int lj_engine_set_profiling(lj_engine* e, int enabled)
 {
     int rc = check_engine(e);
@@ -948,17 +1033,6 @@ int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms
 
 }  // extern "C"
 
-namespace {
-
-struct DevBuf {
-    double* p = nullptr;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
-
-}  // namespace
 
 extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, double* loss,
                                  double* grad_eps, double* grad_sigma, double* grad_pos,

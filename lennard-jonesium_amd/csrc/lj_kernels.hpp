@@ -145,6 +145,7 @@ struct StepArgs {
     int* start;                         // exclusive scan, ncell_total + 1
     int* tile_sum;
     int* perm;                          // arrival-ordered slot lists
+    int* perm_id;                       // particle id of each perm entry
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per block: sum v^2, PE, W, sum d^2
@@ -302,7 +303,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs p)
     const SysDev& S = p.sys[p.blk_sys[b]];
     const int s = b * kBlock + threadIdx.x;
     if (s - S.atom_off >= S.n) return;
-    p.perm[p.start[p.cell_of[s]] + p.slot_of[s]] = s;
+    const int q = p.start[p.cell_of[s]] + p.slot_of[s];
+    p.perm[q] = s;
+    p.perm_id[q] = p.id[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -319,10 +322,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     if (u - S.atom_off >= S.n) return;
     const int old = p.perm[u];
     const int c = p.cell_of[old];
-    const int me = p.id[old];
+    const int me = p.perm_id[u];
     const int c0 = p.start[c], c1 = p.start[c + 1];
     int rank = 0;
-    for (int q = c0; q < c1; ++q) rank += (p.id[p.perm[q]] < me) ? 1 : 0;
+    for (int q = c0; q < c1; ++q) rank += (p.perm_id[q] < me) ? 1 : 0;
     const int dst = c0 + rank;
     const double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
     p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
@@ -617,12 +620,13 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
         double2* sx = st_x + ((PREC == FP64) ? wid * PAD : 0);
         double2* sy = st_y + ((PREC == FP64) ? wid * PAD : 0);
         double2* sz = st_z + ((PREC == FP64) ? wid * PAD : 0);
-        // r^2 < rc^2 and r^2 != 0 (the particle itself) in one unsigned compare
-        const unsigned rc2m1 = __float_as_uint((float)S.rc2) - 1u;
+        const float rc2f = (float)S.rc2;
         const double rc2d = S.rc2;
         const float inv_s2 = (float)(1.0 / S.sigma2);
-        const float ur = (float)(1.0 / (S.sigma2 * S.rc2_red));
-        const float al = (float)S.alpha, be = (float)S.beta;
+        const bool unit_sigma = S.sigma2 == 1.0;
+        const float cv = (float)(S.beta * S.inv_rc2s);          // beta / rc^2 (reduced)
+        const float amb = (float)(S.alpha - S.beta);
+        const float cw = (float)(-2.0 * S.beta * S.inv_rc2s);
 
         // ---- the 27 neighbour cells (reference lattice-image rule), one lane
         //      each: slot range, image shift, and offsets in the concatenation
@@ -670,26 +674,74 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
             int nct = 0;
             const int cap = min(CAP, 64 * nsl);
 
-            for (int base = 0; base < total; base += cap) {
-                // ---- stage neighbours [base, base + m) of the concatenated 27 cells
-                const int m = min(cap, total - base);
-                for (int t = lane; t < m; t += 64) {
-                    const int g = base + t;
-                    int lo_q = 0, hi_q = 26;       // last cell q with off[q] <= g
-                    while (lo_q < hi_q) {
-                        const int mid = (lo_q + hi_q + 1) >> 1;
-                        if (tb_off[wid][mid] <= g) lo_q = mid; else hi_q = mid - 1;
+            // bounding box of this chunk's particles; a neighbour farther than
+            // rc from it cannot interact with any of them
+            double blo[3], bhi[3];
+            {
+                const double c3[3] = {PREC == MIXED ? (double)li.x : xi,
+                                      PREC == MIXED ? (double)li.y : yi,
+                                      PREC == MIXED ? (double)li.z : zi};
+                for (int ax3 = 0; ax3 < 3; ++ax3) {
+                    double lo = act ? c3[ax3] : 1e300, hi = act ? c3[ax3] : -1e300;
+                    for (int o = 32; o > 0; o >>= 1) {
+                        lo = fmin(lo, __shfl_xor(lo, o, 64));
+                        hi = fmax(hi, __shfl_xor(hi, o, 64));
                     }
-                    const int j = tb_start[wid][lo_q] + (g - tb_off[wid][lo_q]);
-                    if (PREC == MIXED) {
-                        const float4 lj = p.lpos[j];
-                        const float4 sh = tb_shf[wid][lo_q];
-                        sf[t] = make_float4(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z, 0.f);
-                    } else {
-                        sx[t] = make_double2(p.pos.x[j], tb_shd[wid][lo_q][0]);
-                        sy[t] = make_double2(p.pos.y[j], tb_shd[wid][lo_q][1]);
-                        sz[t] = make_double2(p.pos.z[j], tb_shd[wid][lo_q][2]);
+                    blo[ax3] = lo;
+                    bhi[ax3] = hi;
+                }
+            }
+            const float flo[3] = {(float)blo[0], (float)blo[1], (float)blo[2]};
+            const float fhi[3] = {(float)bhi[0], (float)bhi[1], (float)bhi[2]};
+            const double keep2 = S.rc2 * (1.0 + 1e-6) + 1e-9;   // conservative margins
+            const float keepf = (float)keep2;
+            const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+            int g0 = 0;
+            while (g0 < total) {
+                // ---- stage the surviving neighbours of the next candidates, 64 at a time
+                int m = 0;
+                while (g0 < total && m <= cap - 64) {
+                    const int g = g0 + lane;
+                    bool keep = false;
+                    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+                    double2 X = make_double2(0, 0), Y = X, Z = X;
+                    if (g < total) {
+                        int lo_q = 0, hi_q = 26;       // last cell q with off[q] <= g
+                        while (lo_q < hi_q) {
+                            const int mid = (lo_q + hi_q + 1) >> 1;
+                            if (tb_off[wid][mid] <= g) lo_q = mid; else hi_q = mid - 1;
+                        }
+                        const int j = tb_start[wid][lo_q] + (g - tb_off[wid][lo_q]);
+                        if (PREC == MIXED) {
+                            const float4 lj = p.lpos[j];
+                            const float4 sh = tb_shf[wid][lo_q];
+                            pf = make_float4(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z, 0.f);
+                            const float ox = fmaxf(fmaxf(flo[0] - pf.x, pf.x - fhi[0]), 0.f);
+                            const float oy = fmaxf(fmaxf(flo[1] - pf.y, pf.y - fhi[1]), 0.f);
+                            const float oz = fmaxf(fmaxf(flo[2] - pf.z, pf.z - fhi[2]), 0.f);
+                            keep = __builtin_fmaf(oz, oz, __builtin_fmaf(oy, oy, ox * ox)) < keepf;
+                        } else {
+                            X = make_double2(p.pos.x[j], tb_shd[wid][lo_q][0]);
+                            Y = make_double2(p.pos.y[j], tb_shd[wid][lo_q][1]);
+                            Z = make_double2(p.pos.z[j], tb_shd[wid][lo_q][2]);
+                            const double q3[3] = {X.x + X.y, Y.x + Y.y, Z.x + Z.y};   // image position
+                            double d2 = 0;
+                            for (int ax3 = 0; ax3 < 3; ++ax3) {
+                                const double o = fmax(fmax(blo[ax3] - q3[ax3], q3[ax3] - bhi[ax3]), 0.0);
+                                d2 += o * o;
+                            }
+                            keep = d2 < keep2;
+                        }
                     }
+                    const unsigned long long bal = __ballot(keep);
+                    if (keep) {
+                        const int t = m + __popcll(bal & below);
+                        if (PREC == MIXED) sf[t] = pf;
+                        else { sx[t] = X; sy[t] = Y; sz[t] = Z; }
+                    }
+                    m += __popcll(bal);
+                    g0 += 64;
                 }
                 // dummy tail: far away, never inside the cutoff
                 if (lane < nsl) {
@@ -710,14 +762,14 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                         const float4 d = sf[q];
                         const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        return (__float_as_uint(r2) - 1u) < rc2m1;
+                        return r2 < rc2f;
                     } else {
                         const double2 X = sx[q], Y = sy[q], Z = sz[q];
                         const double dx = (xi - X.x) - X.y;
                         const double dy = (yi - Y.x) - Y.y;
                         const double dz = (zi - Z.x) - Z.y;
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        return (r2 < rc2d) & (r2 > 0.0);
+                        return r2 < rc2d;
                     }
                 };
                 const int k1 = min(kmax, 32);
@@ -737,13 +789,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                         const float4 d = sf[q];
                         const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        if (r2 == 0.f) { --nct; continue; }   // the particle itself
                         if (LAW == LAW_LJ) {
+                            // s6 = (sigma/r)^6; V = 4 s6^2 - 4 s6 + (alpha - beta) + beta u,
+                            // W = 48 s6^2 - 24 s6 - 2 beta u, u = (r / rc)^2
                             const float inv = __builtin_amdgcn_rcpf(r2);
-                            const float t = inv * inv_s2;
+                            const float t = unit_sigma ? inv : inv * inv_s2;
                             const float s6 = t * t * t;
-                            const float u = r2 * ur;
-                            spe += __builtin_fmaf(4.0f * s6, s6 - 1.0f, __builtin_fmaf(be, u - 1.0f, al));
-                            const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f, -2.0f * be * u);
+                            const float s12 = s6 * s6;
+                            spe += __builtin_fmaf(4.0f, s12, __builtin_fmaf(-4.0f, s6, __builtin_fmaf(r2, cv, amb)));
+                            const float ww = __builtin_fmaf(48.0f, s12, __builtin_fmaf(-24.0f, s6, r2 * cw));
                             sw += ww;
                             const float fm = ww * inv;
                             ax = __builtin_fmaf(fm, dx, ax);
@@ -763,6 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                         const double dy = (yi - Y.x) - Y.y;
                         const double dz = (zi - Z.x) - Z.y;
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        if (r2 == 0.0) { --nct; continue; }   // the particle itself
                         pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
                     }
                 }
@@ -847,8 +903,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
     double acc[4] = {0, 0, 0, 0};
     const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
     const int nb = f.cell_blocks ? S.ncblk : S.nblk;
-    for (int b = threadIdx.x; b < nb; b += kBlock)
-        for (int q = 0; q < 4; ++q) acc[q] += f.partial[4 * (b0 + b) + q];
+#pragma unroll 4
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        const double4 v = reinterpret_cast<const double4*>(f.partial)[b0 + b];
+        acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
+    }
     double out[4];
     block_sum4(acc, out);
     if (threadIdx.x != 0) return;

@@ -84,6 +84,7 @@ def engine():
         "lj_engine_adjoint": (c_i, [V, c_i, c_i, c_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
+        "lj_engine_autotune": (c_i, [V, c_i, P_i, P_d]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),
         "lj_engine_kernel_times": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_char_p), P_d, P_i]),
     }
@@ -105,6 +106,8 @@ def host():
     lib.lj_cubic_lattice.argtypes = [c_i, c_d, c_i, P_d]
     lib.lj_initial_condition.restype = c_d
     lib.lj_initial_condition.argtypes = [c_i, c_d, c_d, ctypes.c_uint, c_i, P_d, P_d]
+    lib.lj_sweep_sizes.restype = c_i
+    lib.lj_sweep_sizes.argtypes = [ctypes.c_uint, c_i, c_i, c_i, P_i]
     lib.lj_format_double.restype = c_i
     lib.lj_format_double.argtypes = [c_d, ctypes.c_char_p, c_i]
     _host = lib

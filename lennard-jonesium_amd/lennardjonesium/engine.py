@@ -221,6 +221,15 @@ class Engine:
         """'tiled' (one wave per home cell, LDS-staged neighbours) or 'particle'."""
         check(self.lib.lj_engine_set_pair_kernel(self.h, PAIR_KERNELS[variant]))
 
+    def autotune(self, steps=20):
+        """Times each pair-kernel variant on this workload and keeps the fastest;
+        returns (variant name, ms per step)."""
+        v = ctypes.c_int()
+        ms = ctypes.c_double()
+        check(self.lib.lj_engine_autotune(self.h, int(steps), ctypes.byref(v), ctypes.byref(ms)))
+        name = {k: n for n, k in PAIR_KERNELS.items()}[v.value]
+        return name, ms.value
+
     def set_profiling(self, enabled):
         check(self.lib.lj_engine_set_profiling(self.h, 1 if enabled else 0))
 
@@ -257,6 +266,16 @@ def cubic_lattice(n, density, lattice=2):
     if L < 0:
         raise ValueError("invalid lattice parameters")
     return pos, L
+
+
+def sweep_sizes(seed, count, lo=5, hi=20):
+    """Replica lattice sizes n_k in [lo, hi] from std::mt19937(seed) with
+    rejection sampling (csrc/host/initial_condition.cpp)."""
+    out = np.zeros(count, dtype=np.int32)
+    if _native.host().lj_sweep_sizes(int(seed) & 0xFFFFFFFF, int(count), int(lo), int(hi),
+                                     out.ctypes.data_as(_native.P_i)) < 0:
+        raise ValueError("invalid range")
+    return out
 
 
 def format_double(x):

@@ -350,3 +350,21 @@ def test_rdf_histogram_matches_brute_force(n, rho):
     assert ns == 1
     assert np.array_equal(counts[0], ref)
     e.close()
+
+
+def test_autotune_leaves_no_trace():
+    eng = _engine_mod()
+    specs, states = [], []
+    for k, (n, rho) in enumerate([(500, 0.9), (2048, 0.1), (864, 0.7)]):
+        pos, vel, L = eng.initial_condition(n, rho, 1.0, seed=k)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((pos, vel))
+    ref = eng.Engine(specs)
+    tuned = eng.Engine(specs)
+    for e in (ref, tuned):
+        for k, (p, v) in enumerate(states):
+            e.set_state(k, p, v)
+    variant, ms = tuned.autotune(4)
+    assert variant in ("tiled", "particle") and ms > 0
+    ref.set_pair_kernel(variant)
+    assert np.array_equal(ref.step(30), tuned.step(30))

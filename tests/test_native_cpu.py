@@ -68,3 +68,14 @@ def test_format_double_reproduces_fixture_lines(golden):
     for line in lines:
         fields = line.split(",")
         assert ",".join([fields[0]] + [format_double(float(x)) for x in fields[1:]]) == line
+
+
+def test_sweep_sizes_deterministic_and_in_range():
+    from lennardjonesium.engine import sweep_sizes
+    a = sweep_sizes(5489, 4096)
+    assert np.array_equal(a, sweep_sizes(5489, 4096))
+    assert a.min() == 5 and a.max() == 20
+    counts = np.bincount(a - 5, minlength=16)
+    assert counts.min() > 180 and counts.max() < 340      # roughly uniform over 16 values
+    # std::mt19937(5489): first output 3499211612 -> 3499211612 % 16 = 12 -> 17
+    assert a[0] == 17
