@@ -549,8 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 //     owning lane writes F, applies the second half kick and accumulates
 //     |v|^2 and |d|^2, and the block sums go to the cell-space partials.
 // ---------------------------------------------------------------------------
-constexpr int kStageMixed = 512;   // staged neighbours per wave (float4 each)
-constexpr int kStageFp64 = 256;    // staged neighbours per wave (x,sx,y,sy,z,sz)
+constexpr int kStageMixed = 256;   // staged neighbours per wave (float4 each)
+constexpr int kStageFp64 = 128;    // staged neighbours per wave (x,sx,y,sy,z,sz)
 
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -589,17 +589,18 @@ __device__ __forceinline__ void pair_mixed(const SysDev& S, float dx, float dy, 
 }
 
 template <int PREC, int LAW, bool KICK>
-__global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
+__global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(StepArgs p)
 {
     constexpr int CAP = (PREC == MIXED) ? kStageMixed : kStageFp64;
     constexpr int PAD = CAP + 64;    // room for the dummy tail read by the last round
     constexpr int NW = kBlock / 64;
     // per-wave staging buffers
-    __shared__ float4 st_f[(PREC == MIXED) ? NW * PAD : 1];
+    __shared__ float3 st_f[(PREC == MIXED) ? NW * PAD : 1];
     __shared__ double2 st_x[(PREC == FP64) ? NW * PAD : 1];
     __shared__ double2 st_y[(PREC == FP64) ? NW * PAD : 1];
     __shared__ double2 st_z[(PREC == FP64) ? NW * PAD : 1];
-    __shared__ double red[NW][64][4];
+    __shared__ double red[(PREC == FP64) ? NW : 1][64][4];
+    __shared__ float redf[(PREC == MIXED) ? NW : 1][64][4];
     __shared__ int tb_start[NW][27], tb_off[NW][27];
     __shared__ float4 tb_shf[(PREC == MIXED) ? NW : 1][27];
     __shared__ double tb_shd[(PREC == FP64) ? NW : 1][27][3];
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
         int cx, cy, cz;
         cell_coords(lc, S.nc, cx, cy, cz);
         const int h0 = p.start[c], h1 = p.start[c + 1];
-        float4* sf = st_f + ((PREC == MIXED) ? wid * PAD : 0);
+        float3* sf = st_f + ((PREC == MIXED) ? wid * PAD : 0);
         double2* sx = st_x + ((PREC == FP64) ? wid * PAD : 0);
         double2* sy = st_y + ((PREC == FP64) ? wid * PAD : 0);
         double2* sz = st_z + ((PREC == FP64) ? wid * PAD : 0);
@@ -676,11 +677,21 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
 
             // bounding box of this chunk's particles; a neighbour farther than
             // rc from it cannot interact with any of them
-            double blo[3], bhi[3];
-            {
-                const double c3[3] = {PREC == MIXED ? (double)li.x : xi,
-                                      PREC == MIXED ? (double)li.y : yi,
-                                      PREC == MIXED ? (double)li.z : zi};
+            double blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+            float flo[3] = {0.f, 0.f, 0.f}, fhi[3] = {0.f, 0.f, 0.f};
+            if (PREC == MIXED) {
+                const float c3[3] = {li.x, li.y, li.z};
+                for (int ax3 = 0; ax3 < 3; ++ax3) {
+                    float lo = act ? c3[ax3] : 3.0e38f, hi = act ? c3[ax3] : -3.0e38f;
+                    for (int o = 32; o > 0; o >>= 1) {
+                        lo = fminf(lo, __shfl_xor(lo, o, 64));
+                        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+                    }
+                    flo[ax3] = lo;
+                    fhi[ax3] = hi;
+                }
+            } else {
+                const double c3[3] = {xi, yi, zi};
                 for (int ax3 = 0; ax3 < 3; ++ax3) {
                     double lo = act ? c3[ax3] : 1e300, hi = act ? c3[ax3] : -1e300;
                     for (int o = 32; o > 0; o >>= 1) {
@@ -691,8 +702,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                     bhi[ax3] = hi;
                 }
             }
-            const float flo[3] = {(float)blo[0], (float)blo[1], (float)blo[2]};
-            const float fhi[3] = {(float)bhi[0], (float)bhi[1], (float)bhi[2]};
             const double keep2 = S.rc2 * (1.0 + 1e-6) + 1e-9;   // conservative margins
             const float keepf = (float)keep2;
             const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -704,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                 while (g0 < total && m <= cap - 64) {
                     const int g = g0 + lane;
                     bool keep = false;
-                    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float3 pf = make_float3(0.f, 0.f, 0.f);
                     double2 X = make_double2(0, 0), Y = X, Z = X;
                     if (g < total) {
                         int lo_q = 0, hi_q = 26;       // last cell q with off[q] <= g
@@ -716,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                         if (PREC == MIXED) {
                             const float4 lj = p.lpos[j];
                             const float4 sh = tb_shf[wid][lo_q];
-                            pf = make_float4(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z, 0.f);
+                            pf = make_float3(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z);
                             const float ox = fmaxf(fmaxf(flo[0] - pf.x, pf.x - fhi[0]), 0.f);
                             const float oy = fmaxf(fmaxf(flo[1] - pf.y, pf.y - fhi[1]), 0.f);
                             const float oz = fmaxf(fmaxf(flo[2] - pf.z, pf.z - fhi[2]), 0.f);
@@ -745,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                 }
                 // dummy tail: far away, never inside the cutoff
                 if (lane < nsl) {
-                    if (PREC == MIXED) sf[m + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+                    if (PREC == MIXED) sf[m + lane] = make_float3(1e18f, 1e18f, 1e18f);
                     else {
                         sx[m + lane] = make_double2(1e150, 0.0);
                         sy[m + lane] = make_double2(1e150, 0.0);
@@ -759,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                 auto test = [&](int kb) -> bool {
                     const int q = sl + kb * nsl;
                     if (PREC == MIXED) {
-                        const float4 d = sf[q];
+                        const float3 d = sf[q];
                         const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         return r2 < rc2f;
@@ -786,7 +795,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                     mask &= mask - 1;
                     const int q = sl + kb * nsl;
                     if (PREC == MIXED) {
-                        const float4 d = sf[q];
+                        const float3 d = sf[q];
                         const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         if (r2 == 0.f) { --nct; continue; }   // the particle itself
@@ -827,26 +836,38 @@ __global__ __launch_bounds__(kBlock) void k_pair_cells(StepArgs p)
                 w += (double)sw;
                 wave_lds_sync();
             }
-            if (PREC == MIXED) {
-                if (LAW == LAW_LJ && S.eps != 1.0) {
-                    ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
-                    pe *= S.eps; w *= S.eps;
-                }
-                fx = ax; fy = ay; fz = az;
+            if (PREC == MIXED && LAW == LAW_LJ && S.eps != 1.0) {
+                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
+                pe *= S.eps; w *= S.eps;
             }
             // ---- sum the slices of each particle in a fixed order
-            red[wid][lane][0] = fx;
-            red[wid][lane][1] = fy;
-            red[wid][lane][2] = fz;
-            red[wid][lane][3] = (double)nct;
+            if (PREC == MIXED) {
+                redf[wid][lane][0] = ax;
+                redf[wid][lane][1] = ay;
+                redf[wid][lane][2] = az;
+                redf[wid][lane][3] = __int_as_float(nct);
+            } else {
+                red[wid][lane][0] = fx;
+                red[wid][lane][1] = fy;
+                red[wid][lane][2] = fz;
+                red[wid][lane][3] = (double)nct;
+            }
             wave_lds_sync();
             if (act && sl == 0) {
                 for (int t = 1; t < nsl; ++t) {
-                    fx += red[wid][il + t * ni][0];
-                    fy += red[wid][il + t * ni][1];
-                    fz += red[wid][il + t * ni][2];
-                    nct += (int)red[wid][il + t * ni][3];
+                    if (PREC == MIXED) {
+                        ax += redf[wid][il + t * ni][0];
+                        ay += redf[wid][il + t * ni][1];
+                        az += redf[wid][il + t * ni][2];
+                        nct += __float_as_int(redf[wid][il + t * ni][3]);
+                    } else {
+                        fx += red[wid][il + t * ni][0];
+                        fy += red[wid][il + t * ni][1];
+                        fz += red[wid][il + t * ni][2];
+                        nct += (int)red[wid][il + t * ni][3];
+                    }
                 }
+                if (PREC == MIXED) { fx = ax; fy = ay; fz = az; }
                 if (p.ncount) p.ncount[si] = nct;
                 p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
