@@ -368,3 +368,29 @@ def test_autotune_leaves_no_trace():
     assert variant in ("tiled", "particle") and ms > 0
     ref.set_pair_kernel(variant)
     assert np.array_equal(ref.step(30), tuned.step(30))
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_clumped_cell_chunks_and_stage_splits(kernel):
+    """300 particles inside one cell of a 4^3 grid: the tiled kernel walks
+    several 64-particle chunks and splits its neighbour stages; results must
+    still match the oracle (FP64 counts bit-exact, forces 1e-12)."""
+    eng = _engine_mod()
+    rng = np.random.default_rng(23)
+    n = 300
+    pos = rng.uniform(0.1, 2.4, size=(n, 3))
+    pos[:40] = rng.uniform(0, 10.0, size=(40, 3))      # some spread-out particles too
+    e = make(eng, n, 10.0, pair_kernel=kernel)
+    e.set_state(0, pos, np.zeros_like(pos))
+    assert np.array_equal(e.neighbor_counts(0), _oracle_counts(pos, 10.0))
+    F = e.get_state(0)["force"]
+    Fr, per, wr = ljref.forces(pos, 10.0)
+    assert force_err(F, Fr) < FTOL
+    e.close()
+    e = make(eng, n, 10.0, precision="mixed", pair_kernel=kernel)
+    e.set_state(0, pos, np.zeros_like(pos))
+    got = e.neighbor_counts(0)
+    idx, sep = ljref.pairs_cell(pos, 10.0, 2.5)
+    near = (np.abs(np.sqrt((sep ** 2).sum(1)) - 2.5) < 1e-5).sum()
+    assert np.abs(got - _oracle_counts(pos, 10.0)).sum() <= 2 * near + 4
+    e.close()
