@@ -9,6 +9,7 @@
 
 #include "lj_engine.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,6 +112,11 @@ struct lj_engine {
 
     bool profiling = false;
     std::vector<KernelTimer> timers;
+    // Profiling events: pairs (2k, 2k+1) bracket the k-th pending launch;
+    // pending[k] is its timer slot.  Read back only when flushed.
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> pending;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // events of the launch being timed
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ljk::StepArgs args(int from) const
@@ -169,7 +175,28 @@ struct lj_engine {
 
 namespace {
 
-// Launch helper: optional per-kernel HIP-event timing (eager launches only).
+// Per-kernel timing: while profiling, every launch carries a start and a stop
+// event of its own (hipExtLaunchKernel), which HIP stamps at the kernel's own
+// start and end, so a bracket is the kernel alone, without the dispatch gap
+// in front of it.  No host wait sits between launches; flush_timers() waits
+// once and accumulates the brackets.
+constexpr int kMaxPendingTimers = 2048;
+
+int flush_timers(lj_engine* e)
+{
+    if (e->pending.empty()) return LJ_OK;
+    HIP_TRY(hipEventSynchronize(e->prof_ev[2 * (e->pending.size() - 1) + 1]));
+    for (size_t k = 0; k < e->pending.size(); ++k) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->prof_ev[2 * k], e->prof_ev[2 * k + 1]));
+        KernelTimer& t = e->timers[e->pending[k]];
+        t.total_ms += ms;
+        t.calls += 1;
+    }
+    e->pending.clear();
+    return LJ_OK;
+}
+
 template <class F>
 int timed(lj_engine* e, int slot, const char* name, F&& launch)
 {
@@ -179,15 +206,34 @@ int timed(lj_engine* e, int slot, const char* name, F&& launch)
     }
     if ((int)e->timers.size() <= slot) e->timers.resize(slot + 1);
     e->timers[slot].name = name;
-    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    if ((int)e->pending.size() == kMaxPendingTimers) {
+        int rc = flush_timers(e);
+        if (rc) return rc;
+    }
+    const size_t k = e->pending.size();
+    while (e->prof_ev.size() < 2 * k + 2) {
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreate(&ev));
+        e->prof_ev.push_back(ev);
+    }
+    e->ev_start = e->prof_ev[2 * k];
+    e->ev_stop = e->prof_ev[2 * k + 1];
     launch();
-    HIP_TRY(hipEventRecord(e->ev1, e->stream));
-    HIP_TRY(hipEventSynchronize(e->ev1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    e->timers[slot].total_ms += ms;
-    e->timers[slot].calls += 1;
+    e->ev_start = e->ev_stop = nullptr;
+    HIP_TRY(hipGetLastError());
+    e->pending.push_back(slot);
     return LJ_OK;
+}
+
+// Launches kernel k on the engine stream, with the timing events when a
+// timed() bracket is open.
+template <typename... Params, typename... Args>
+void launch(lj_engine* e, void (*k)(Params...), dim3 grid, dim3 block, Args... args)
+{
+    if (e->ev_start)
+        hipExtLaunchKernelGGL(k, grid, block, 0, e->stream, e->ev_start, e->ev_stop, 0, args...);
+    else
+        hipLaunchKernelGGL(k, grid, block, 0, e->stream, args...);
 }
 
 // Cell build on the current positions: optional integration (first half
@@ -200,32 +246,32 @@ int enqueue_cells(lj_engine* e, bool integrate)
     int rc = LJ_OK;
     if (integrate)
         rc = timed(e, 0, "integrate_key", [&] {
-            hipLaunchKernelGGL((ljk::k_integrate_key<true, true>), gb, tb, 0, e->stream, p);
+            launch(e, ljk::k_integrate_key<true, true>, gb, tb, p);
         });
     else
         rc = timed(e, 0, "integrate_key", [&] {
-            hipLaunchKernelGGL((ljk::k_integrate_key<false, true>), gb, tb, 0, e->stream, p);
+            launch(e, ljk::k_integrate_key<false, true>, gb, tb, p);
         });
     if (rc) return rc;
     if ((rc = timed(e, 1, "scan_tiles", [&] {
-             hipLaunchKernelGGL(ljk::k_scan_tiles, dim3(e->ntile), tb, 0, e->stream, p);
+             launch(e, ljk::k_scan_tiles, dim3(e->ntile), tb, p);
          })))
         return rc;
     if ((rc = timed(e, 2, "scan_apply", [&] {
-             hipLaunchKernelGGL(ljk::k_scan_apply, dim3(e->ntile), tb, 0, e->stream, p);
+             launch(e, ljk::k_scan_apply, dim3(e->ntile), tb, p);
          })))
         return rc;
     if ((rc = timed(e, 3, "scatter", [&] {
-             hipLaunchKernelGGL(ljk::k_scatter, gb, tb, 0, e->stream, p);
+             launch(e, ljk::k_scatter, gb, tb, p);
          })))
         return rc;
     if (e->precision == LJ_PRECISION_MIXED)
         rc = timed(e, 4, "sort_reorder", [&] {
-            hipLaunchKernelGGL((ljk::k_sort_reorder<true>), gb, tb, 0, e->stream, p);
+            launch(e, ljk::k_sort_reorder<true>, gb, tb, p);
         });
     else
         rc = timed(e, 4, "sort_reorder", [&] {
-            hipLaunchKernelGGL((ljk::k_sort_reorder<false>), gb, tb, 0, e->stream, p);
+            launch(e, ljk::k_sort_reorder<false>, gb, tb, p);
         });
     if (rc) return rc;
     e->cur ^= 1;
@@ -236,15 +282,16 @@ int enqueue_cells(lj_engine* e, bool integrate)
 template <int PREC, int LAW, bool KICK>
 void launch_pair(lj_engine* e, const ljk::StepArgs& p)
 {
-    hipLaunchKernelGGL((ljk::k_pair<PREC, LAW, KICK>), dim3(e->nblk), dim3(kBlock), 0, e->stream,
-                       p);
+    launch(e, ljk::k_pair<PREC, LAW, KICK>, dim3(e->nblk), dim3(kBlock), p);
 }
 
 template <int PREC, int LAW, bool KICK>
 void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
-    hipLaunchKernelGGL((ljk::k_pair_cells<PREC, LAW, KICK>), dim3(e->ncblk), dim3(kBlock), 0,
-                       e->stream, p);
+    if constexpr (PREC == ljk::MIXED)
+        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
+    else
+        launch(e, ljk::k_pair_cells<PREC, LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
 }
 
 template <int LAW, bool KICK>
@@ -285,7 +332,7 @@ int enqueue_finalize(lj_engine* e, int mode)
 {
     const ljk::FinalArgs f = e->fargs(mode);
     return timed(e, 6, "finalize", [&] {
-        hipLaunchKernelGGL(ljk::k_finalize, dim3(e->nsys), dim3(kBlock), 0, e->stream, f);
+        launch(e, ljk::k_finalize, dim3(e->nsys), dim3(kBlock), f);
     });
 }
 
@@ -298,8 +345,7 @@ int enqueue_step(lj_engine* e)
     } else {
         const ljk::StepArgs p = e->args(e->cur);
         if ((rc = timed(e, 0, "integrate_key", [&] {
-                 hipLaunchKernelGGL((ljk::k_integrate_key<true, false>), dim3(e->nblk),
-                                    dim3(kBlock), 0, e->stream, p);
+                 launch(e, ljk::k_integrate_key<true, false>, dim3(e->nblk), dim3(kBlock), p);
              })))
             return rc;
     }
@@ -575,6 +621,7 @@ int lj_engine_destroy(lj_engine* e)
         for (void* m : more)
             if (m) (void)hipFree(m);
     }
+    for (hipEvent_t ev : e->prof_ev) (void)hipEventDestroy(ev);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1009,6 +1056,7 @@ int lj_engine_set_profiling(lj_engine* e, int enabled)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = flush_timers(e))) return rc;
     e->profiling = enabled != 0;
     e->timers.clear();
     return LJ_OK;
@@ -1018,6 +1066,7 @@ int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = flush_timers(e))) return rc;
     int k = 0;
     for (const KernelTimer& t : e->timers) {
         if (!t.calls) continue;

@@ -416,10 +416,24 @@ __device__ __forceinline__ void pair_fp64(const SysDev& S, double dx, double dy,
     w += ww;
 }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  Renumbering so that XCD x walks one
+// contiguous range of logical blocks keeps each XCD on one slab of
+// neighbouring cells, so a cell's neighbours are fetched into one L2 instead
+// of all eight.  Bijective on [0, gridDim.x).
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_slab_block()
+{
+    const int b = blockIdx.x, n = gridDim.x;
+    const int x = b % kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 template <int PREC, int LAW, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_slab_block();
     const SysDev& S = p.sys[p.blk_sys[b]];
     const int s = b * kBlock + threadIdx.x;
     const bool active = (s - S.atom_off) < S.n && (!KICK || S.active);
@@ -605,7 +619,7 @@ __global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(St
     __shared__ float4 tb_shf[(PREC == MIXED) ? NW : 1][27];
     __shared__ double tb_shd[(PREC == FP64) ? NW : 1][27][3];
 
-    const int b = blockIdx.x;
+    const int b = xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -887,6 +901,303 @@ __global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(St
                 wacc[2] += (1. / 2.) * w;
             }
             wave_lds_sync();
+        }
+    }
+    double out[4];
+    block_sum4(wacc, out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6c. tiled pair kernel, mixed precision: one wave64 per home cell, with the
+//     cutoff test reduced to three FMAs per candidate.
+//
+//   * coordinates are float, relative to the centre o of the chunk's bounding
+//     box (the cell-local lpos minus the neighbour cell's image shift minus
+//     o), so |x| stays below a few sigma;
+//   * staged neighbours are stored slice-major as (r, |r|^2): slice s holds
+//     staged entries s, s + S, s + 2S, ... in consecutive rows, so a lane
+//     walks its rows at immediate LDS offsets;
+//   * phase 1 evaluates |x_i - r|^2 < rc^2 as |r|^2 - 2 x_i.r < rc^2 - |x_i|^2
+//     and shifts the outcome into a 64-bit mask; the entry of the particle
+//     itself is recorded while staging and its bit is cleared, so phase 2
+//     runs the pair law on every set bit without a test;
+//   * phase 2 accumulates the force and, per stage, sum s^12, sum s^6 and
+//     sum r^2, from which V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u and
+//     W = 48 s^12 - 24 s^6 - 2 beta u (u = r^2 / rc^2) follow in double.
+// ---------------------------------------------------------------------------
+constexpr int kStageM = 256;   // staged candidates per stage
+constexpr int kRowsM = 324;    // >= max over slice counts S of S * rows(S) + 3
+
+template <int LAW, bool KICK>
+__global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ float4 st[NW][kRowsM];
+    __shared__ int tb_start[NW][27];
+    __shared__ int tb_off[NW][32];      // entries 27..31 = INT_MAX (search padding)
+    __shared__ float4 tb_sh[NW][27];    // image shift + chunk origin
+    __shared__ int selfe[NW][64];       // staged entry of each chunk particle (-1: none)
+
+    const int b = xcd_slab_block();
+    const int k = p.cblk_sys[b];
+    const SysDev& S = p.sys[k];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
+    double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
+
+    if (lc < S.ncell && (!KICK || S.active)) {
+        const int c = S.cell_off + lc;
+        int cx, cy, cz;
+        cell_coords(lc, S.nc, cx, cy, cz);
+        const int h0 = p.start[c], h1 = p.start[c + 1];
+        float4* sw = st[wid];
+        const float rc2f = (float)S.rc2;
+        const float inv_s2 = (float)(1.0 / S.sigma2);
+        const float cw = (float)(-2.0 * S.beta * S.inv_rc2s);
+        const float keepf = rc2f + 1e-3f;   // bounding-box pruning, wider than any rounding
+
+        // ---- the 27 neighbour cells (reference lattice-image rule), one lane each
+        int cntq = 0;
+        float shx = 0.f, shy = 0.f, shz = 0.f;
+        if (lane < 27) {
+            const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+            int nbx, ix, nby, iy, nbz, iz;
+            nb_axis(cx, ox, S.nc[0], nbx, ix);
+            nb_axis(cy, oy, S.nc[1], nby, iy);
+            nb_axis(cz, oz, S.nc[2], nbz, iz);
+            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+            const int j0 = p.start[nb];
+            cntq = p.start[nb + 1] - j0;
+            tb_start[wid][lane] = j0;
+            shx = (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]);
+            shy = (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]);
+            shz = (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]);
+        }
+        const int incl = wave_incl_scan(cntq);
+        if (lane < 27) tb_off[wid][lane] = incl - cntq;
+        else if (lane < 32) tb_off[wid][lane] = 0x7fffffff;
+        const int total = __shfl(incl, 26, 64);
+        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+        for (int i0 = h0; i0 < h1; i0 += 64) {
+            const int ni = min(64, h1 - i0);
+            const int nsl = 64 / ni;
+            const bool act = lane < nsl * ni;
+            const int il = act ? lane % ni : 0;
+            const int sl = act ? lane / ni : 0;
+            const int si = i0 + il;
+            const float4 li = p.lpos[si];
+
+            // bounding box of the chunk, its centre o and half extents h
+            float o3[3], h3[3];
+            {
+                const float c3[3] = {li.x, li.y, li.z};
+#pragma unroll
+                for (int a3 = 0; a3 < 3; ++a3) {
+                    float lo = act ? c3[a3] : 3.0e38f, hi = act ? c3[a3] : -3.0e38f;
+                    for (int o = 32; o > 0; o >>= 1) {
+                        lo = fminf(lo, __shfl_xor(lo, o, 64));
+                        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+                    }
+                    o3[a3] = 0.5f * (lo + hi);
+                    h3[a3] = 0.5f * (hi - lo);
+                }
+            }
+            const float xi = li.x - o3[0], yi = li.y - o3[1], zi = li.z - o3[2];
+            const float m2x = -2.0f * xi, m2y = -2.0f * yi, m2z = -2.0f * zi;
+            const float thr = rc2f - __builtin_fmaf(zi, zi, __builtin_fmaf(yi, yi, xi * xi));
+
+            // slice-major stage geometry (uniform)
+            const int cap = min(kStageM, 64 * nsl);
+            int SL = (cap + nsl - 1) / nsl;
+            if (!(SL & 1) && nsl * (SL + 1) + 3 <= kRowsM) ++SL;   // odd row count
+            const unsigned magic = (1u << 20) / (unsigned)nsl + 1u;  // t / nsl = t * magic >> 20
+            const float4* rowp = sw + sl * SL;
+
+            wave_lds_sync();   // the previous chunk's reads of tb_sh / sw are done
+            if (lane < 27)
+                tb_sh[wid][lane] = make_float4(shx + o3[0], shy + o3[1], shz + o3[2], 0.f);
+            wave_lds_sync();
+
+            // candidate g of the concatenated neighbour cells: its cell q, slot j
+            // and relative position (prefetched one round ahead)
+            auto fetch = [&](int g, int& q, int& j, float4& v) {
+                q = 0; j = 0;
+                v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (g < total) {
+                    int pos = 0;
+#pragma unroll
+                    for (int step = 16; step > 0; step >>= 1)
+                        if (tb_off[wid][pos + step] <= g) pos += step;
+                    q = pos;
+                    j = tb_start[wid][q] + (g - tb_off[wid][q]);
+                    v = p.lpos[j];
+                }
+            };
+            int nq, nj;
+            float4 nv;
+            fetch(lane, nq, nj, nv);
+            int g0 = 0;
+            double pe = 0.0, w = 0.0;
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            int nct = 0;
+            while (g0 < total) {
+                // ---- stage the surviving neighbours of the next candidates, 64 at a time
+                if (lane < ni) selfe[wid][lane] = -1;
+                int m = 0;
+                while (g0 < total && m <= cap - 64) {
+                    const int cq = nq, cj = nj;
+                    const float4 cv = nv;
+                    const bool valid = g0 + lane < total;
+                    g0 += 64;
+                    fetch(g0 + lane, nq, nj, nv);
+                    bool keep = false;
+                    float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (valid) {
+                        const float4 sh = tb_sh[wid][cq];
+                        rv.x = cv.x - sh.x;
+                        rv.y = cv.y - sh.y;
+                        rv.z = cv.z - sh.z;
+                        const float ex = fmaxf(fabsf(rv.x) - h3[0], 0.f);
+                        const float ey = fmaxf(fabsf(rv.y) - h3[1], 0.f);
+                        const float ez = fmaxf(fabsf(rv.z) - h3[2], 0.f);
+                        keep = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < keepf;
+                        rv.w = __builtin_fmaf(rv.z, rv.z, __builtin_fmaf(rv.y, rv.y, rv.x * rv.x));
+                    }
+                    const unsigned long long bal = __ballot(keep);
+                    if (keep) {
+                        const int t = m + __popcll(bal & below);
+                        const int row = (int)(((unsigned)t * magic) >> 20);
+                        const int e = (t - row * nsl) * SL + row;
+                        sw[e] = rv;
+                        if (cq == 13 && (unsigned)(cj - i0) < (unsigned)ni) selfe[wid][cj - i0] = e;
+                    }
+                    m += __popcll(bal);
+                }
+                const int kmax = __builtin_amdgcn_readfirstlane(
+                    (int)(((unsigned)(m + nsl - 1) * magic) >> 20));
+                // far-away fillers for the unused tail of the last row
+                if (lane < kmax * nsl - m) {
+                    const int t = m + lane;
+                    const int row = (int)(((unsigned)t * magic) >> 20);
+                    sw[(t - row * nsl) * SL + row] = make_float4(1e18f, 1e18f, 1e18f, 3e36f);
+                }
+                wave_lds_sync();
+                if (kmax == 0) continue;
+
+                // ---- phase 1: kmax rows per lane, four per step (rows past kmax are dropped)
+                const int k4 = (kmax + 3) & ~3;
+                const int nlo = min(k4, 32), nhi = k4 - nlo;
+                unsigned lo = 0u, hi = 0u;
+                for (int kb = 0; kb < nlo; kb += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float4 d = rowp[kb + u];
+                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
+                        lo = lo + lo + (v < thr ? 1u : 0u);
+                    }
+                }
+                for (int kb = 32; kb < 32 + nhi; kb += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float4 d = rowp[kb + u];
+                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
+                        hi = hi + hi + (v < thr ? 1u : 0u);
+                    }
+                }
+                // row kb -> bit 63 - kb
+                lo = (nlo == 32) ? lo : (lo << (32 - nlo));
+                hi = (nhi == 0) ? 0u : ((nhi == 32) ? hi : (hi << (32 - nhi)));
+                unsigned long long mask = (((unsigned long long)lo << 32) | hi) & (~0ull << (64 - kmax));
+                const int self_row = selfe[wid][il] - sl * SL;
+                if ((unsigned)self_row < (unsigned)SL) mask &= ~(1ull << (63 - self_row));
+                if (!act) mask = 0ull;
+                const int hits = __popcll(mask);
+                nct += hits;
+
+                // ---- phase 2: pair law on every set bit
+                const float4* top = rowp + 63;
+                if (LAW == LAW_LJ) {
+                    float s12s = 0.f, s6s = 0.f, r2s = 0.f;
+                    while (mask) {
+                        const int bit = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float4 d = top[-bit];
+                        const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float inv = __builtin_amdgcn_rcpf(r2);
+                        const float t = inv * inv_s2;
+                        const float s6 = t * t * t;
+                        const float s12 = s6 * s6;
+                        s12s += s12;
+                        s6s += s6;
+                        r2s += r2;
+                        const float ww = __builtin_fmaf(48.0f, s12, __builtin_fmaf(-24.0f, s6, r2 * cw));
+                        const float fm = ww * inv;
+                        ax = __builtin_fmaf(fm, dx, ax);
+                        ay = __builtin_fmaf(fm, dy, ay);
+                        az = __builtin_fmaf(fm, dz, az);
+                    }
+                    const double u_sum = (double)r2s * S.inv_rc2s;
+                    pe += 4.0 * (double)s12s - 4.0 * (double)s6s + (S.alpha - S.beta) * (double)hits +
+                          S.beta * u_sum;
+                    w += 48.0 * (double)s12s - 24.0 * (double)s6s - 2.0 * S.beta * u_sum;
+                } else {
+                    float spe = 0.f, sww = 0.f;
+                    while (mask) {
+                        const int bit = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float4 d = top[-bit];
+                        const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float nrm = sqrtf(r2);
+                        const float ww = (float)S.strength * nrm;
+                        spe += (float)S.strength * ((float)S.rc - nrm);
+                        sww += ww;
+                        const float fm = ww / r2;
+                        ax += fm * dx; ay += fm * dy; az += fm * dz;
+                    }
+                    pe += (double)spe;
+                    w += (double)sww;
+                }
+                wave_lds_sync();
+            }
+            if (LAW == LAW_LJ && S.eps != 1.0) {
+                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
+                pe *= S.eps; w *= S.eps;
+            }
+            // ---- sum the slices of each particle in a fixed order
+            sw[lane] = make_float4(ax, ay, az, __int_as_float(nct));
+            wave_lds_sync();
+            if (act && sl == 0) {
+                for (int t = 1; t < nsl; ++t) {
+                    const float4 r = sw[il + t * ni];
+                    ax += r.x; ay += r.y; az += r.z;
+                    nct += __float_as_int(r.w);
+                }
+                const double fx = ax, fy = ay, fz = az;
+                if (p.ncount) p.ncount[si] = nct;
+                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
+                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+                if (KICK) {
+                    const double dt = S.dt;
+                    vx = vx + ((1. / 2.) * fx) * dt;
+                    vy = vy + ((1. / 2.) * fy) * dt;
+                    vz = vz + ((1. / 2.) * fz) * dt;
+                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
+                }
+                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                wacc[0] += (vx * vx + vy * vy) + vz * vz;
+                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
+            }
+            if (act) {
+                wacc[1] += (1. / 2.) * pe;
+                wacc[2] += (1. / 2.) * w;
+            }
         }
     }
     double out[4];
