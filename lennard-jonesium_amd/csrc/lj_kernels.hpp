@@ -60,6 +60,11 @@ struct SysDev {
     double eps, sigma2;
     double strength, rc;   // constant law
     double dt;
+    // mixed-precision forms of the above (filled by the host)
+    float rc2f;        // (float) rc2
+    float inv_s2f;     // (float) (1 / sigma2)
+    float cwf;         // (float) (-2 beta inv_rc2s)
+    float pad_f;
 };
 
 // ---------------------------------------------------------------------------
@@ -932,12 +937,56 @@ __global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(St
 constexpr int kStageM = 256;   // staged candidates per stage
 constexpr int kRowsM = 324;    // >= max over slice counts S of S * rows(S) + 3
 
+// Stage geometry of a chunk of ni particles (uniform per wave, looked up).
+struct SliceGeom {
+    int nsl;          // slices: 64 / ni
+    int rows;         // rows per slice in the stage buffer (odd where it fits)
+    unsigned magic;   // t / nsl == (t * magic) >> 20 for 0 <= t < 2048
+    int cap;          // staged candidates per stage: min(kStageM, 64 nsl)
+    unsigned magic_ni;   // lane / ni == (lane * magic_ni) >> 20 for lane < 64
+};
+
+constexpr SliceGeom slice_geom(int ni)
+{
+    const int nsl = 64 / ni;
+    const int cap = kStageM < 64 * nsl ? kStageM : 64 * nsl;
+    int rows = (cap + nsl - 1) / nsl;
+    if (!(rows & 1) && nsl * (rows + 1) + 3 <= kRowsM) ++rows;
+    return SliceGeom{nsl, rows, (1u << 20) / (unsigned)nsl + 1u, cap,
+                     (1u << 20) / (unsigned)ni + 1u};
+}
+
+struct SliceGeomTable {
+    SliceGeom g[65];
+    constexpr SliceGeomTable() : g{}
+    {
+        for (int ni = 1; ni <= 64; ++ni) g[ni] = slice_geom(ni);
+    }
+};
+
+__constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
+
+// Wave-wide minimum (MAX = false) or maximum of an int: four in-row DPP steps
+// (quad swaps, half-row and row mirrors), then the two cross-row exchanges.
+template <bool MAX>
+__device__ __forceinline__ int wave_extreme(int v)
+{
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));    // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));    // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));   // row_half_mirror
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));   // row_mirror
+    v = op(v, __shfl_xor(v, 16, 64));
+    v = op(v, __shfl_xor(v, 32, 64));
+    return v;
+}
+
 template <int LAW, bool KICK>
 __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
 {
     constexpr int NW = kBlock / 64;
     __shared__ float4 st[NW][kRowsM];
-    __shared__ int tb_start[NW][27];
+    __shared__ int tb_delta[NW][27];    // slot of a cell's first particle minus its offset
     __shared__ int tb_off[NW][32];      // entries 27..31 = INT_MAX (search padding)
     __shared__ float4 tb_sh[NW][27];    // image shift + chunk origin
     __shared__ int selfe[NW][64];       // staged entry of each chunk particle (-1: none)
@@ -945,7 +994,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     const int b = xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
@@ -953,15 +1003,16 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
         const int c = S.cell_off + lc;
         int cx, cy, cz;
         cell_coords(lc, S.nc, cx, cy, cz);
-        const int h0 = p.start[c], h1 = p.start[c + 1];
+        const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
+        const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
         float4* sw = st[wid];
-        const float rc2f = (float)S.rc2;
-        const float inv_s2 = (float)(1.0 / S.sigma2);
-        const float cw = (float)(-2.0 * S.beta * S.inv_rc2s);
+        const float rc2f = S.rc2f;
+        const float inv_s2 = S.inv_s2f;
+        const float cw = S.cwf;
         const float keepf = rc2f + 1e-3f;   // bounding-box pruning, wider than any rounding
 
         // ---- the 27 neighbour cells (reference lattice-image rule), one lane each
-        int cntq = 0;
+        int cntq = 0, j0 = 0;
         float shx = 0.f, shy = 0.f, shz = 0.f;
         if (lane < 27) {
             const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
@@ -970,39 +1021,46 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             nb_axis(cy, oy, S.nc[1], nby, iy);
             nb_axis(cz, oz, S.nc[2], nbz, iz);
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
-            const int j0 = p.start[nb];
+            j0 = p.start[nb];
             cntq = p.start[nb + 1] - j0;
-            tb_start[wid][lane] = j0;
             shx = (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]);
             shy = (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]);
             shz = (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]);
         }
         const int incl = wave_incl_scan(cntq);
-        if (lane < 27) tb_off[wid][lane] = incl - cntq;
-        else if (lane < 32) tb_off[wid][lane] = 0x7fffffff;
-        const int total = __shfl(incl, 26, 64);
+        if (lane < 27) {
+            tb_off[wid][lane] = incl - cntq;
+            tb_delta[wid][lane] = j0 - (incl - cntq);
+        } else if (lane < 32) {
+            tb_off[wid][lane] = 0x7fffffff;
+        }
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 26, 64));
+        // offset of the home cell (offset 13) in the concatenation
+        const int off_home = __builtin_amdgcn_readfirstlane(__shfl(incl - cntq, 13, 64));
         const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
         for (int i0 = h0; i0 < h1; i0 += 64) {
             const int ni = min(64, h1 - i0);
-            const int nsl = 64 / ni;
+            const SliceGeom G = kSliceGeom.g[ni];
+            const int nsl = G.nsl;
             const bool act = lane < nsl * ni;
-            const int il = act ? lane % ni : 0;
-            const int sl = act ? lane / ni : 0;
+            const int q_ni = (int)(((unsigned)lane * G.magic_ni) >> 20);   // lane / ni
+            const int il = act ? lane - q_ni * ni : 0;
+            const int sl = act ? q_ni : 0;
             const int si = i0 + il;
             const float4 li = p.lpos[si];
 
             // bounding box of the chunk, its centre o and half extents h
+            // (cell-local coordinates are >= 0 up to rounding residues of
+            // ~1e-14, so their bit patterns order as signed ints)
             float o3[3], h3[3];
             {
                 const float c3[3] = {li.x, li.y, li.z};
 #pragma unroll
                 for (int a3 = 0; a3 < 3; ++a3) {
-                    float lo = act ? c3[a3] : 3.0e38f, hi = act ? c3[a3] : -3.0e38f;
-                    for (int o = 32; o > 0; o >>= 1) {
-                        lo = fminf(lo, __shfl_xor(lo, o, 64));
-                        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
-                    }
+                    const int bits = __float_as_int(c3[a3]);
+                    const float lo = __int_as_float(wave_extreme<false>(act ? bits : 0x7fffffff));
+                    const float hi = __int_as_float(wave_extreme<true>(act ? bits : (int)0x80000000));
                     o3[a3] = 0.5f * (lo + hi);
                     h3[a3] = 0.5f * (hi - lo);
                 }
@@ -1012,10 +1070,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             const float thr = rc2f - __builtin_fmaf(zi, zi, __builtin_fmaf(yi, yi, xi * xi));
 
             // slice-major stage geometry (uniform)
-            const int cap = min(kStageM, 64 * nsl);
-            int SL = (cap + nsl - 1) / nsl;
-            if (!(SL & 1) && nsl * (SL + 1) + 3 <= kRowsM) ++SL;   // odd row count
-            const unsigned magic = (1u << 20) / (unsigned)nsl + 1u;  // t / nsl = t * magic >> 20
+            const int cap = G.cap;
+            const int SL = G.rows;
+            const unsigned magic = G.magic;   // t / nsl = t * magic >> 20
             const float4* rowp = sw + sl * SL;
 
             wave_lds_sync();   // the previous chunk's reads of tb_sh / sw are done
@@ -1023,26 +1080,26 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 tb_sh[wid][lane] = make_float4(shx + o3[0], shy + o3[1], shz + o3[2], 0.f);
             wave_lds_sync();
 
-            // candidate g of the concatenated neighbour cells: its cell q, slot j
-            // and relative position (prefetched one round ahead)
-            auto fetch = [&](int g, int& q, int& j, float4& v) {
-                q = 0; j = 0;
-                v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (g < total) {
-                    int pos = 0;
+            // candidate g of the concatenated neighbour cells (clamped to the
+            // last one, so every lane loads): its cell q and cell-local position,
+            // fetched one round ahead
+            auto fetch = [&](int g, int& q, float4& v) {
+                const int gc = min(g, total - 1);
+                int pos = 0;
 #pragma unroll
-                    for (int step = 16; step > 0; step >>= 1)
-                        if (tb_off[wid][pos + step] <= g) pos += step;
-                    q = pos;
-                    j = tb_start[wid][q] + (g - tb_off[wid][q]);
-                    v = p.lpos[j];
-                }
+                for (int step = 16; step > 0; step >>= 1)
+                    if (tb_off[wid][pos + step] <= gc) pos += step;
+                q = pos;
+                const unsigned j = (unsigned)(tb_delta[wid][pos] + gc);
+                v = *(const float4*)((const char*)p.lpos + j * 16u);
             };
-            int nq, nj;
+            int nq;
             float4 nv;
-            fetch(lane, nq, nj, nv);
+            fetch(lane, nq, nv);
+            const int self0 = off_home + (i0 - h0);   // concatenation index of the chunk's first particle
             int g0 = 0;
-            double pe = 0.0, w = 0.0;
+            double pe = 0.0, w = 0.0;              // constant law
+            double s12a = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of s^12, s^6, r^2
             float ax = 0.f, ay = 0.f, az = 0.f;
             int nct = 0;
             while (g0 < total) {
@@ -1050,31 +1107,31 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 if (lane < ni) selfe[wid][lane] = -1;
                 int m = 0;
                 while (g0 < total && m <= cap - 64) {
-                    const int cq = nq, cj = nj;
+                    const int cq = nq;
                     const float4 cv = nv;
-                    const bool valid = g0 + lane < total;
+                    const int g = g0 + lane;
                     g0 += 64;
-                    fetch(g0 + lane, nq, nj, nv);
-                    bool keep = false;
-                    float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (valid) {
-                        const float4 sh = tb_sh[wid][cq];
-                        rv.x = cv.x - sh.x;
-                        rv.y = cv.y - sh.y;
-                        rv.z = cv.z - sh.z;
-                        const float ex = fmaxf(fabsf(rv.x) - h3[0], 0.f);
-                        const float ey = fmaxf(fabsf(rv.y) - h3[1], 0.f);
-                        const float ez = fmaxf(fabsf(rv.z) - h3[2], 0.f);
-                        keep = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < keepf;
-                        rv.w = __builtin_fmaf(rv.z, rv.z, __builtin_fmaf(rv.y, rv.y, rv.x * rv.x));
-                    }
+                    fetch(g0 + lane, nq, nv);
+                    const float4 sh = tb_sh[wid][cq];
+                    float4 rv;
+                    rv.x = cv.x - sh.x;
+                    rv.y = cv.y - sh.y;
+                    rv.z = cv.z - sh.z;
+                    const float ex = fmaxf(fabsf(rv.x) - h3[0], 0.f);
+                    const float ey = fmaxf(fabsf(rv.y) - h3[1], 0.f);
+                    const float ez = fmaxf(fabsf(rv.z) - h3[2], 0.f);
+                    const bool keep =
+                        g < total && __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < keepf;
+                    rv.w = __builtin_fmaf(rv.z, rv.z, __builtin_fmaf(rv.y, rv.y, rv.x * rv.x));
                     const unsigned long long bal = __ballot(keep);
                     if (keep) {
-                        const int t = m + __popcll(bal & below);
-                        const int row = (int)(((unsigned)t * magic) >> 20);
-                        const int e = (t - row * nsl) * SL + row;
+                        const unsigned t = (unsigned)m + (unsigned)__popcll(bal & below);
+                        const unsigned row = (t * magic) >> 20;
+                        const unsigned e = (t - row * (unsigned)nsl) * (unsigned)SL + row;
                         sw[e] = rv;
-                        if (cq == 13 && (unsigned)(cj - i0) < (unsigned)ni) selfe[wid][cj - i0] = e;
+                        // chunk particles occupy [self0, self0 + ni) of the home cell's range
+                        const unsigned ic = (unsigned)(g - self0);
+                        if (ic < (unsigned)ni) selfe[wid][ic] = (int)e;
                     }
                     m += __popcll(bal);
                 }
@@ -1116,8 +1173,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 const int self_row = selfe[wid][il] - sl * SL;
                 if ((unsigned)self_row < (unsigned)SL) mask &= ~(1ull << (63 - self_row));
                 if (!act) mask = 0ull;
-                const int hits = __popcll(mask);
-                nct += hits;
+                nct += __popcll(mask);
 
                 // ---- phase 2: pair law on every set bit
                 const float4* top = rowp + 63;
@@ -1142,10 +1198,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                         ay = __builtin_fmaf(fm, dy, ay);
                         az = __builtin_fmaf(fm, dz, az);
                     }
-                    const double u_sum = (double)r2s * S.inv_rc2s;
-                    pe += 4.0 * (double)s12s - 4.0 * (double)s6s + (S.alpha - S.beta) * (double)hits +
-                          S.beta * u_sum;
-                    w += 48.0 * (double)s12s - 24.0 * (double)s6s - 2.0 * S.beta * u_sum;
+                    s12a += (double)s12s;
+                    s6a += (double)s6s;
+                    r2a += (double)r2s;
                 } else {
                     float spe = 0.f, sww = 0.f;
                     while (mask) {
@@ -1166,9 +1221,15 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 }
                 wave_lds_sync();
             }
-            if (LAW == LAW_LJ && S.eps != 1.0) {
-                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
-                pe *= S.eps; w *= S.eps;
+            if (LAW == LAW_LJ) {
+                // V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u, W = 48 s^12 - 24 s^6 - 2 beta u
+                const double u_sum = r2a * S.inv_rc2s;
+                pe = 4.0 * s12a - 4.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum;
+                w = 48.0 * s12a - 24.0 * s6a - 2.0 * S.beta * u_sum;
+                if (S.eps != 1.0) {
+                    ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
+                    pe *= S.eps; w *= S.eps;
+                }
             }
             // ---- sum the slices of each particle in a fixed order
             sw[lane] = make_float4(ax, ay, az, __int_as_float(nct));
