@@ -95,6 +95,7 @@ struct lj_engine {
     double* d_partial = nullptr;
     double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
     int* d_step = nullptr;
+    int* d_pending = nullptr;   // per system: a step awaits its finalize
     double* d_records = nullptr;
     int rec_cap = 0;
     double* d_measure = nullptr;
@@ -148,6 +149,7 @@ struct lj_engine {
         p.scell = d_scell;
         p.lpos = d_lpos;
         p.partial = d_partial;
+        p.pending = d_pending;
         p.ncount = d_ncount;
         p.ncell_total = ncell_total;
         p.nslot = nslot;
@@ -169,7 +171,16 @@ struct lj_engine {
         f.cap = rec_cap;
         f.mode = mode;
         f.cell_blocks = last_pair_cells ? 1 : 0;
+        f.pending = d_pending;
+        if (mode == 0) f.cell_blocks = step_uses_cells() ? 1 : 0;
         return f;
+    }
+
+    // A step's pair kernel is the tiled one (cell-space block sums) unless the
+    // law has no pair force or the per-particle variant is selected.
+    bool step_uses_cells() const
+    {
+        return sys[0].law != ljk::LAW_NONE && pair_variant == LJ_PAIR_CELL_TILED;
     }
 };
 
@@ -236,6 +247,8 @@ void launch(lj_engine* e, void (*k)(Params...), dim3 grid, dim3 block, Args... a
         hipLaunchKernelGGL(k, grid, block, 0, e->stream, args...);
 }
 
+int enqueue_finalize(lj_engine* e, int mode);
+
 // Cell build on the current positions: optional integration (first half
 // kick, drift, wrap) fused with the keys, then scan, scatter and the sorted
 // move into the other buffer set.  Leaves e->cur pointing at the sorted set.
@@ -253,14 +266,25 @@ int enqueue_cells(lj_engine* e, bool integrate)
             launch(e, ljk::k_integrate_key<false, true>, gb, tb, p);
         });
     if (rc) return rc;
-    if ((rc = timed(e, 1, "scan_tiles", [&] {
-             launch(e, ljk::k_scan_tiles, dim3(e->ntile), tb, p);
-         })))
-        return rc;
-    if ((rc = timed(e, 2, "scan_apply", [&] {
-             launch(e, ljk::k_scan_apply, dim3(e->ntile), tb, p);
-         })))
-        return rc;
+    if (e->ncell_total <= ljk::kScanOneMax * 4 * ljk::kScanOneThreads) {
+        // a step also finishes the previous step's records (pending flags decide)
+        const ljk::FinalArgs f = e->fargs(0);
+        const int nfin = integrate ? e->nsys : 0;
+        if ((rc = timed(e, 1, "scan", [&] {
+                 launch(e, ljk::k_scan_one, dim3(1 + nfin), dim3(ljk::kScanOneThreads), p, f);
+             })))
+            return rc;
+    } else {
+        if (integrate && (rc = enqueue_finalize(e, 0))) return rc;
+        if ((rc = timed(e, 1, "scan_tiles", [&] {
+                 launch(e, ljk::k_scan_tiles, dim3(e->ntile), tb, p);
+             })))
+            return rc;
+        if ((rc = timed(e, 2, "scan_apply", [&] {
+                 launch(e, ljk::k_scan_apply, dim3(e->ntile), tb, p);
+             })))
+            return rc;
+    }
     if ((rc = timed(e, 3, "scatter", [&] {
              launch(e, ljk::k_scatter, gb, tb, p);
          })))
@@ -336,7 +360,10 @@ int enqueue_finalize(lj_engine* e, int mode)
     });
 }
 
-// One velocity-Verlet step.
+// One velocity-Verlet step.  Its record is written during the next step's
+// cell build (or before the next step's integration without cells) or by
+// enqueue_finalize(e, 0), which must follow the last step of a sequence
+// before anything else reads records, time or the block sums.
 int enqueue_step(lj_engine* e)
 {
     int rc;
@@ -344,13 +371,13 @@ int enqueue_step(lj_engine* e)
         if ((rc = enqueue_cells(e, true))) return rc;
     } else {
         const ljk::StepArgs p = e->args(e->cur);
+        if ((rc = enqueue_finalize(e, 0))) return rc;
         if ((rc = timed(e, 0, "integrate_key", [&] {
                  launch(e, ljk::k_integrate_key<true, false>, dim3(e->nblk), dim3(kBlock), p);
              })))
             return rc;
     }
-    if ((rc = enqueue_pair(e, true, false))) return rc;
-    return enqueue_finalize(e, 0);
+    return enqueue_pair(e, true, false);
 }
 
 int ensure_records(lj_engine* e, int nsteps)
@@ -556,6 +583,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_partial, 4 * (size_t)std::max(blk, cblk))) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
+        (rc = dev_alloc(&e->d_pending, nsys)) ||
         (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
         return cleanup_fail(rc);
     for (int b = 0; b < 2; ++b) {
@@ -590,7 +618,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
               h2d(e->d_force, zeros.data(), 3 * ns * sizeof(double)) &&
               h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
-              h2d(e->d_vir, zeros.data(), nsys * sizeof(double));
+              h2d(e->d_vir, zeros.data(), nsys * sizeof(double)) &&
+              h2d(e->d_pending, std::vector<int>(nsys, 0).data(), nsys * sizeof(int));
     for (int b = 0; b < 2 && ok; ++b)
         ok = h2d(e->d_pos[b], zeros.data(), 3 * ns * sizeof(double)) &&
              h2d(e->d_vel[b], zeros.data(), 3 * ns * sizeof(double)) &&
@@ -615,7 +644,7 @@ int lj_engine_destroy(lj_engine* e)
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_perm_id,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_start, e->d_tile,
-                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_records,
+                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_records,
                     e->d_measure, e->d_rdf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -728,6 +757,7 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
         if ((rc = enqueue_step(e))) return rc;
         ++done;
     }
+    if ((rc = enqueue_finalize(e, 0))) return rc;   // the last step's record
     HIP_TRY(hipGetLastError());
     if (records)
         HIP_TRY(hipMemcpyAsync(records, e->d_records, (size_t)nsteps * e->nsys * sizeof(lj_thermo),
@@ -1170,6 +1200,7 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
             if ((rc = enqueue_step(e))) return rc;
             to_id(seg_pos.p + (size_t)k * n3, seg_vel.p + (size_t)k * n3);
         }
+        if ((rc = enqueue_finalize(e, 0))) return rc;
         for (int k = t1 - t0; k >= 1; --k) {
             const double* vt = seg_vel.p + (size_t)k * n3;
             hipLaunchKernelGGL(ljk::k_adj_pre, gv, tb, 0, e->stream, n3, 0.5 * dt, cT, lam_v.p, vt,

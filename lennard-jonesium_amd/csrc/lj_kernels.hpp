@@ -77,24 +77,48 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Sums four per-thread doubles over a kBlock-thread block; thread 0 gets them.
+// One DPP lane move of a double (two 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Wave-wide sum in a fixed order: in-row butterflies over DPP (quad swaps,
+// half-row and row mirrors), then the four row sums.  Uniform result.
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_f64<0xB1>(v);    // quad_perm 1,0,3,2
+    v += dpp_f64<0x4E>(v);    // quad_perm 2,3,0,1
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Sums four per-thread doubles over an NT-thread block; thread 0 gets them.
+template <int NT = kBlock>
 __device__ __forceinline__ void block_sum4(double v[4], double out[4])
 {
-    __shared__ double red[4][kBlock / 64];
+    constexpr int NW = NT / 64;
+    __shared__ double red[4][NW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        double s = wave_sum(v[k]);
+        const double s = wave_sum_dpp(v[k]);
         if (lane == 0) red[k][wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double s = 0;
-            for (int w = 0; w < kBlock / 64; ++w) s += red[k][w];
-            out[k] = s;
-        }
+        for (int k = 0; k < 4; ++k) out[k] = wave_sum_dpp(lane < NW ? red[k][lane] : 0.0);
     }
 }
 
@@ -154,10 +178,34 @@ struct StepArgs {
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per block: sum v^2, PE, W, sum d^2
+    int* pending;                       // per system: a step's block sums await the finalize
     int* ncount;                        // optional: in-cutoff neighbours per slot
     int ncell_total;
     int nslot;
 };
+
+// ---------------------------------------------------------------------------
+// per-system reduction parameters (the reduction itself is section 7)
+//   mode 0: after a step; 1: after a force-only calculation; 2: measure only
+// ---------------------------------------------------------------------------
+struct FinalArgs {
+    const SysDev* sys;
+    const double* partial;
+    double* time;      // per system
+    double* pe;        // per system (last force calculation)
+    double* vir;
+    int* step;         // per system record counter
+    double* records;   // [cap][nsys][7]
+    double* measure;   // [nsys][7] (mode 2)
+    int nsys;
+    int cap;
+    int mode;
+    int cell_blocks;   // partial sums come from cell-space blocks (tiled pair kernel)
+    int* pending;      // per system: set by a step's pair kernel, cleared here
+};
+
+template <int NT>
+__device__ void finalize_system(const FinalArgs& f, int k);
 
 // ---------------------------------------------------------------------------
 // 1. kick + drift + wrap + cell key + cell count
@@ -238,6 +286,18 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
+// The same inclusive scan from DPP row shifts and row broadcasts (no LDS).
+__device__ __forceinline__ int wave_incl_scan_dpp(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
 // Exclusive block scan of one int per thread (kBlock threads); *total gets the sum.
 __device__ __forceinline__ int block_excl_scan(int v, int* total)
 {
@@ -299,6 +359,78 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p)
     }
 }
 
+// Single-block form for up to kScanOneMax * 4096 cells: 1024 threads load
+// all their int4 tiles first (one memory latency), scan every tile's waves
+// side by side, and one wave combines the wave sums into offsets: two block
+// barriers in all.  The counts are zeroed later, by k_sort_reorder.
+// Blocks 1.. of the launch finish the previous step's records instead
+// (section 7), in parallel with the scan.
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOneMax = 8;
+
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalArgs f)
+{
+    if (blockIdx.x > 0) {
+        finalize_system<kScanOneThreads>(f, blockIdx.x - 1);
+        return;
+    }
+    constexpr int NW = kScanOneThreads / 64;
+    __shared__ int wsum[kScanOneMax][NW];
+    const int n = p.ncell_total;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int c[kScanOneMax][4], v[kScanOneMax], inc[kScanOneMax];
+#pragma unroll
+    for (int t = 0; t < kScanOneMax; ++t) {
+        // one 16-byte load per tile (the buffers are 16-byte aligned); the
+        // ragged end falls back to single words
+        const int base = t * 4 * kScanOneThreads + 4 * threadIdx.x;
+        if (base + 3 < n) {
+            const int4 q = *reinterpret_cast<const int4*>(p.count + base);
+            c[t][0] = q.x; c[t][1] = q.y; c[t][2] = q.z; c[t][3] = q.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[t][k] = (base + k < n) ? p.count[base + k] : 0;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kScanOneMax; ++t) {
+        v[t] = c[t][0] + c[t][1] + c[t][2] + c[t][3];
+        inc[t] = wave_incl_scan_dpp(v[t]);
+        if (lane == 63) wsum[t][wid] = inc[t];
+    }
+    __syncthreads();
+    if (wid == 0) {
+        // lane w < NW: exclusive offsets of wave w in every tile, plus the
+        // running total of the preceding tiles
+        int carry = 0;
+#pragma unroll
+        for (int t = 0; t < kScanOneMax; ++t) {
+            const int x = lane < NW ? wsum[t][lane] : 0;
+            const int xi = wave_incl_scan_dpp(x);
+            const int tile_tot = __builtin_amdgcn_readlane(xi, NW - 1);
+            if (lane < NW) wsum[t][lane] = carry + xi - x;
+            carry += tile_tot;
+        }
+        if (lane == 0) p.start[n] = carry;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kScanOneMax; ++t) {
+        const int ex = wsum[t][wid] + inc[t] - v[t];
+        const int base = t * 4 * kScanOneThreads + 4 * threadIdx.x;
+        const int4 q = make_int4(ex, ex + c[t][0], ex + c[t][0] + c[t][1],
+                                 ex + c[t][0] + c[t][1] + c[t][2]);
+        if (base + 3 < n) {
+            *reinterpret_cast<int4*>(p.start + base) = q;
+        } else {
+            const int e4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (base + k < n) p.start[base + k] = e4[k];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 4. scatter: slot lists per cell in arrival order
 // ---------------------------------------------------------------------------
@@ -329,6 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     const int c = p.cell_of[old];
     const int me = p.perm_id[u];
     const int c0 = p.start[c], c1 = p.start[c + 1];
+    if (u == c0) p.count[c] = 0;   // ready for the next counting pass (tail cells keep theirs)
     int rank = 0;
     for (int q = c0; q < c1; ++q) rank += (p.perm_id[q] < me) ? 1 : 0;
     const int dst = c0 + rank;
@@ -546,6 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) p.partial[4 * b + k] = out[k];
+        if (KICK && b == S.blk_first) p.pending[p.blk_sys[b]] = 1;   // the step awaits its finalize
     }
 }
 
@@ -913,6 +1047,7 @@ __global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(St
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
 
@@ -1155,7 +1290,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                     for (int u = 0; u < 4; ++u) {
                         const float4 d = rowp[kb + u];
                         const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
-                        lo = lo + lo + (v < thr ? 1u : 0u);
+                        // shift in the sign of v - thr (set iff v < thr)
+                        lo = __builtin_amdgcn_alignbit(lo, (unsigned)__float_as_int(v - thr), 31);
                     }
                 }
                 for (int kb = 32; kb < 32 + nhi; kb += 4) {
@@ -1163,7 +1299,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                     for (int u = 0; u < 4; ++u) {
                         const float4 d = rowp[kb + u];
                         const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
-                        hi = hi + hi + (v < thr ? 1u : 0u);
+                        // shift in the sign of v - thr (set iff v < thr)
+                        hi = __builtin_amdgcn_alignbit(hi, (unsigned)__float_as_int(v - thr), 31);
                     }
                 }
                 // row kb -> bit 63 - kb
@@ -1266,6 +1403,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
 
@@ -1274,37 +1412,29 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
 //    (physics/measurements.hpp:59-70; time += dt as engine/integrator.cpp:85)
 //    mode 0: after a step; 1: after a force-only calculation; 2: measure only
 // ---------------------------------------------------------------------------
-struct FinalArgs {
-    const SysDev* sys;
-    const double* partial;
-    double* time;      // per system
-    double* pe;        // per system (last force calculation)
-    double* vir;
-    int* step;         // per system record counter
-    double* records;   // [cap][nsys][7]
-    double* measure;   // [nsys][7] (mode 2)
-    int nsys;
-    int cap;
-    int mode;
-    int cell_blocks;   // partial sums come from cell-space blocks (tiled pair kernel)
-};
 
-__global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
+// Reduces system k's block sums into its record (one NT-thread block).
+// Mode 0 runs only while the system's step is pending and clears the flag.
+template <int NT>
+__device__ void finalize_system(const FinalArgs& f, int k)
 {
-    const int k = blockIdx.x;
     const SysDev& S = f.sys[k];
+    const int pend = f.mode == 0 ? f.pending[k] : 1;
     double acc[4] = {0, 0, 0, 0};
     const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
-    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
-#pragma unroll 4
-    for (int b = threadIdx.x; b < nb; b += kBlock) {
+    const int nb = pend ? (f.cell_blocks ? S.ncblk : S.nblk) : 0;
+#pragma unroll 8
+    for (int b = threadIdx.x; b < nb; b += NT) {
         const double4 v = reinterpret_cast<const double4*>(f.partial)[b0 + b];
         acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
     }
     double out[4];
-    block_sum4(acc, out);
-    if (threadIdx.x != 0) return;
-    if (f.mode == 0 && !S.active) return;
+    block_sum4<NT>(acc, out);
+    if (threadIdx.x != 0 || !pend) return;
+    if (f.mode == 0) {
+        f.pending[k] = 0;
+        if (!S.active) return;
+    }
     double pe = f.pe[k], w = f.vir[k];
     if (f.mode != 2 && S.law != LAW_NONE) {
         pe = out[1];
@@ -1336,6 +1466,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
     r[4] = w;
     r[5] = (2. / 3.) * ke / (double)S.n;
     r[6] = out[3] / (double)S.n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
+{
+    finalize_system<kBlock>(f, blockIdx.x);
 }
 
 __global__ void k_reset_steps(int* step, int nsys)
