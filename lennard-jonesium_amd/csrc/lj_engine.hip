@@ -249,9 +249,11 @@ void launch(lj_engine* e, void (*k)(Params...), dim3 grid, dim3 block, Args... a
 
 int enqueue_finalize(lj_engine* e, int mode);
 
-// Cell build on the current positions: optional integration (first half
-// kick, drift, wrap) fused with the keys, then scan, scatter and the sorted
-// move into the other buffer set.  Leaves e->cur pointing at the sorted set.
+// Cell build, optionally for the integrated positions (first half kick,
+// drift, wrap): the key pass evaluates the integrated position for the keys
+// only, and the sorted move into the other buffer set applies the same
+// integration while it moves each particle.  Between them: scan, scatter.
+// Leaves e->cur pointing at the sorted set.
 int enqueue_cells(lj_engine* e, bool integrate)
 {
     const ljk::StepArgs p = e->args(e->cur);
@@ -259,11 +261,11 @@ int enqueue_cells(lj_engine* e, bool integrate)
     int rc = LJ_OK;
     if (integrate)
         rc = timed(e, 0, "integrate_key", [&] {
-            launch(e, ljk::k_integrate_key<true, true>, gb, tb, p);
+            launch(e, ljk::k_integrate_key<2, true>, gb, tb, p);
         });
     else
         rc = timed(e, 0, "integrate_key", [&] {
-            launch(e, ljk::k_integrate_key<false, true>, gb, tb, p);
+            launch(e, ljk::k_integrate_key<0, true>, gb, tb, p);
         });
     if (rc) return rc;
     if (e->ncell_total <= ljk::kScanOneMax * 4 * ljk::kScanOneThreads) {
@@ -289,14 +291,13 @@ int enqueue_cells(lj_engine* e, bool integrate)
              launch(e, ljk::k_scatter, gb, tb, p);
          })))
         return rc;
-    if (e->precision == LJ_PRECISION_MIXED)
-        rc = timed(e, 4, "sort_reorder", [&] {
-            launch(e, ljk::k_sort_reorder<true>, gb, tb, p);
-        });
-    else
-        rc = timed(e, 4, "sort_reorder", [&] {
-            launch(e, ljk::k_sort_reorder<false>, gb, tb, p);
-        });
+    const bool mixed = e->precision == LJ_PRECISION_MIXED;
+    rc = timed(e, 4, "sort_reorder", [&] {
+        if (mixed && integrate) launch(e, ljk::k_sort_reorder<true, true>, gb, tb, p);
+        else if (mixed) launch(e, ljk::k_sort_reorder<true, false>, gb, tb, p);
+        else if (integrate) launch(e, ljk::k_sort_reorder<false, true>, gb, tb, p);
+        else launch(e, ljk::k_sort_reorder<false, false>, gb, tb, p);
+    });
     if (rc) return rc;
     e->cur ^= 1;
     e->sorted = true;
@@ -373,7 +374,7 @@ int enqueue_step(lj_engine* e)
         const ljk::StepArgs p = e->args(e->cur);
         if ((rc = enqueue_finalize(e, 0))) return rc;
         if ((rc = timed(e, 0, "integrate_key", [&] {
-                 launch(e, ljk::k_integrate_key<true, false>, dim3(e->nblk), dim3(kBlock), p);
+                 launch(e, ljk::k_integrate_key<1, false>, dim3(e->nblk), dim3(kBlock), p);
              })))
             return rc;
     }

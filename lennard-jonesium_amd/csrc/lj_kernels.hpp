@@ -211,7 +211,32 @@ __device__ void finalize_system(const FinalArgs& f, int k);
 // 1. kick + drift + wrap + cell key + cell count
 //    engine/integrator.cpp:69-78 and engine/particle_pair_filter.cpp:141-154
 // ---------------------------------------------------------------------------
-template <bool INTEGRATE, bool KEYS>
+// First half kick, drift and periodic wrap of one particle (velocity Verlet,
+// engine/integrator.cpp:69-78, engine/boundary_condition.cpp:45-48):
+// v += (1/2) F dt with the stored force (zero before the first calculation),
+// then r += v dt and d += v dt (the same increment).  The cell-key pass and
+// the sort evaluate it identically.
+__device__ __forceinline__ void advance(const SysDev& S, double& x, double& y, double& z,
+                                        double& vx, double& vy, double& vz, double fx, double fy,
+                                        double fz, double& ix, double& iy, double& iz)
+{
+    const double dt = S.dt;
+    vx = vx + ((1. / 2.) * fx) * dt;
+    vy = vy + ((1. / 2.) * fy) * dt;
+    vz = vz + ((1. / 2.) * fz) * dt;
+    ix = vx * dt; iy = vy * dt; iz = vz * dt;
+    x = x + ix; y = y + iy; z = z + iz;
+    if (S.periodic) {
+        x = x - floor(x / S.L[0]) * S.L[0];
+        y = y - floor(y / S.L[1]) * S.L[1];
+        z = z - floor(z / S.L[2]) * S.L[2];
+    }
+}
+
+// MODE 0: keys of the current positions; 1: integrate in place (and key);
+// 2: key of the integrated position only -- the sort applies the step while
+// it moves the particle, so the state is read once and written once.
+template <int MODE, bool KEYS>
 __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
 {
     const int b = blockIdx.x;
@@ -221,25 +246,17 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
     int c = -1;
     if (active) {
         double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
-        if (INTEGRATE && S.active) {
-            const double dt = S.dt;
-            // v += (1/2) F dt   (the stored force: zero before the first calculation)
-            double vx = p.vel.x[s] + ((1. / 2.) * p.force.x[s]) * dt;
-            double vy = p.vel.y[s] + ((1. / 2.) * p.force.y[s]) * dt;
-            double vz = p.vel.z[s] + ((1. / 2.) * p.force.z[s]) * dt;
-            // r += v dt ; d += v dt  (same increment)
-            double ix = vx * dt, iy = vy * dt, iz = vz * dt;
-            x = x + ix; y = y + iy; z = z + iz;
-            p.disp.x[s] = p.disp.x[s] + ix;
-            p.disp.y[s] = p.disp.y[s] + iy;
-            p.disp.z[s] = p.disp.z[s] + iz;
-            if (S.periodic) {   // engine/boundary_condition.cpp:45-48
-                x = x - floor(x / S.L[0]) * S.L[0];
-                y = y - floor(y / S.L[1]) * S.L[1];
-                z = z - floor(z / S.L[2]) * S.L[2];
+        if (MODE != 0 && S.active) {
+            double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+            double ix, iy, iz;
+            advance(S, x, y, z, vx, vy, vz, p.force.x[s], p.force.y[s], p.force.z[s], ix, iy, iz);
+            if (MODE == 1) {
+                p.disp.x[s] = p.disp.x[s] + ix;
+                p.disp.y[s] = p.disp.y[s] + iy;
+                p.disp.z[s] = p.disp.z[s] + iz;
+                p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+                p.pos.x[s] = x; p.pos.y[s] = y; p.pos.z[s] = z;
             }
-            p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
-            p.pos.x[s] = x; p.pos.y[s] = y; p.pos.z[s] = z;
         }
         if (KEYS) {
             const int kx = cell_axis(x, S.q[0], S.nc[0]);
@@ -450,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs p)
 //    sorted slots (next buffers); mixed precision also writes cell-local
 //    float coordinates.
 // ---------------------------------------------------------------------------
-template <bool MIXED_>
+template <bool MIXED_, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
 {
     const int b = blockIdx.x;
@@ -465,11 +482,17 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     int rank = 0;
     for (int q = c0; q < c1; ++q) rank += (p.perm_id[q] < me) ? 1 : 0;
     const int dst = c0 + rank;
-    const double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
+    double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
+    double vx = p.vel.x[old], vy = p.vel.y[old], vz = p.vel.z[old];
+    double dx = p.disp.x[old], dy = p.disp.y[old], dz = p.disp.z[old];
+    if (INTEGRATE && S.active) {
+        double ix, iy, iz;
+        advance(S, x, y, z, vx, vy, vz, p.force.x[old], p.force.y[old], p.force.z[old], ix, iy, iz);
+        dx = dx + ix; dy = dy + iy; dz = dz + iz;
+    }
     p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
-    p.vel_n.x[dst] = p.vel.x[old]; p.vel_n.y[dst] = p.vel.y[old]; p.vel_n.z[dst] = p.vel.z[old];
-    p.disp_n.x[dst] = p.disp.x[old]; p.disp_n.y[dst] = p.disp.y[old];
-    p.disp_n.z[dst] = p.disp.z[old];
+    p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
+    p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
     p.id_n[dst] = me;
     p.scell[dst] = c;
     if (MIXED_) {
