@@ -92,6 +92,7 @@ struct lj_engine {
     int* d_perm_id = nullptr;
     float4* d_lpos = nullptr;
     int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
+    int* d_cell_xyz = nullptr;   // per cell: x | y << 10 | z << 20
     double* d_partial = nullptr;
     double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
     int* d_step = nullptr;
@@ -150,6 +151,7 @@ struct lj_engine {
         p.lpos = d_lpos;
         p.partial = d_partial;
         p.pending = d_pending;
+        p.cell_xyz = d_cell_xyz;
         p.ncount = d_ncount;
         p.ncell_total = ncell_total;
         p.nslot = nslot;
@@ -539,6 +541,16 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
                 s.a[a] = d.box[a] / (double)s.nc[a];
             }
             s.ncell = s.nc[0] * s.nc[1] * s.nc[2];
+            for (int a = 0; a < 3; ++a) {
+                if (s.nc[a] > 1023) {
+                    delete e;
+                    return fail(LJ_E_INVALID, "more than 1023 cells along a box side");
+                }
+                for (int o = -1; o <= 1; ++o)
+                    for (int i = -1; i <= 1; ++i)
+                        s.shf[a][3 * (o + 1) + (i + 1)] =
+                            (float)((i * s.nc[a] - o) * s.a[a] - i * s.L[a]);
+            }
         }
         const int slots = round_up(s.n, kBlock);
         s.atom_off = atom_off;
@@ -598,6 +610,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             (rc = dev_alloc(&e->d_scell, ns)) ||
             (rc = dev_alloc(&e->d_count, e->ncell_total)) ||
             (rc = dev_alloc(&e->d_count_init, e->ncell_total)) ||
+            (rc = dev_alloc(&e->d_cell_xyz, e->ncell_total)) ||
             (rc = dev_alloc(&e->d_start, e->ncell_total + 1)) ||
             (rc = dev_alloc(&e->d_tile, e->ntile)))
             return cleanup_fail(rc);
@@ -627,7 +640,15 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
              h2d(e->d_disp[b], zeros.data(), 3 * ns * sizeof(double)) &&
              h2d(e->d_id[b], ident.data(), ns * sizeof(int));
     if (ok && e->has_cells) {
-        ok = h2d(e->d_count, count_init.data(), e->ncell_total * sizeof(int)) &&
+        std::vector<int> xyz(e->ncell_total, 0);
+        for (const SysDev& sd : e->sys)
+            for (int lc = 0; lc < sd.ncell; ++lc) {
+                const int cz = lc % sd.nc[2], cy = (lc / sd.nc[2]) % sd.nc[1];
+                const int cx = lc / (sd.nc[2] * sd.nc[1]);
+                xyz[sd.cell_off + lc] = cx | (cy << 10) | (cz << 20);
+            }
+        ok = h2d(e->d_cell_xyz, xyz.data(), e->ncell_total * sizeof(int)) &&
+             h2d(e->d_count, count_init.data(), e->ncell_total * sizeof(int)) &&
              h2d(e->d_count_init, count_init.data(), e->ncell_total * sizeof(int));
     }
     if (!ok) return cleanup_fail(fail(LJ_E_HIP, "initial upload failed"));
@@ -644,7 +665,7 @@ int lj_engine_destroy(lj_engine* e)
     if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_perm_id,
-                    e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_start, e->d_tile,
+                    e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
                     e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_records,
                     e->d_measure, e->d_rdf};
     for (void* b : bufs)

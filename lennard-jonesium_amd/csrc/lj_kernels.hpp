@@ -65,6 +65,9 @@ struct SysDev {
     float inv_s2f;     // (float) (1 / sigma2)
     float cwf;         // (float) (-2 beta inv_rc2s)
     float pad_f;
+    // image shift of a neighbour cell, per axis: shf[a][3 (o + 1) + (i + 1)] =
+    // (float) ((i nc - o) a - i L) for step o and lattice image i in {-1, 0, 1}
+    float shf[3][9];
 };
 
 // ---------------------------------------------------------------------------
@@ -132,12 +135,13 @@ __device__ __forceinline__ int cell_axis(double r, double q, int nc)
     return k >= nc ? nc - 1 : k;
 }
 
-__device__ __forceinline__ void cell_coords(int lc, const int nc[3], int& cx, int& cy, int& cz)
+// Integer coordinates of global cell c from the per-cell table.
+__device__ __forceinline__ void cell_xyz_of(const int* tab, int c, int& cx, int& cy, int& cz)
 {
-    cz = lc % nc[2];
-    int t = lc / nc[2];
-    cy = t % nc[1];
-    cx = t / nc[1];
+    const int v = tab[c];
+    cx = v & 1023;
+    cy = (v >> 10) & 1023;
+    cz = v >> 20;
 }
 
 // Neighbour cell along one axis for step o in {-1, 0, 1}: the reference's
@@ -179,6 +183,7 @@ struct StepArgs {
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per block: sum v^2, PE, W, sum d^2
     int* pending;                       // per system: a step's block sums await the finalize
+    const int* cell_xyz;                // per cell: x | y << 10 | z << 20 (cell coordinates)
     int* ncount;                        // optional: in-cutoff neighbours per slot
     int ncell_total;
     int nslot;
@@ -497,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     p.scell[dst] = c;
     if (MIXED_) {
         int cx, cy, cz;
-        cell_coords(c - S.cell_off, S.nc, cx, cy, cz);
+        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         p.lpos[dst] = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
                                   (float)(z - cz * S.a[2]), 0.0f);
     }
@@ -518,7 +523,7 @@ __device__ __forceinline__ void for_each_neighbour_fp64(const SysDev& S, const S
                                                         int s, Visit&& visit)
 {
     int cx, cy, cz;
-    cell_coords(p.scell[s] - S.cell_off, S.nc, cx, cy, cz);
+    cell_xyz_of(p.cell_xyz, p.scell[s], cx, cy, cz);
     const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
     const double rc2 = S.rc2;
     for (int ox = -1; ox <= 1; ++ox) {
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
             // cell-local float coordinates; sep = (l_i - l_j) + shift, where
             // shift = (image * nc - step) * a - image * L for the neighbour cell
             int cx, cy, cz;
-            cell_coords(p.scell[s] - S.cell_off, S.nc, cx, cy, cz);
+            cell_xyz_of(p.cell_xyz, p.scell[s], cx, cy, cz);
             const float4 li = p.lpos[s];
             const float rc2f = (float)S.rc2;
             const float inv_s2 = (float)(1.0 / S.sigma2);
@@ -735,6 +740,45 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Block sums (kBlock threads) of four per-lane doubles through an LDS
+// transpose: sc is 256 doubles of LDS owned by the calling wave.  Lane l adds
+// the four entries 4 (l % 16) .. + 3 of quantity l / 16, a 16-lane DPP
+// reduction completes each quantity within one row, and wave 0 adds the
+// waves' results with a quad DPP reduction.  Thread 0 gets the sums.
+__device__ __forceinline__ void block_sum4_lds(const double v[4], double* sc, double out[4])
+{
+    constexpr int NW = kBlock / 64;
+    static_assert(NW == 4, "one quad of waves");
+    __shared__ double red[4][NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sc[q * 64 + lane] = v[q];
+    wave_lds_sync();
+    const double* row = sc + (lane >> 4) * 64 + 4 * (lane & 15);
+    double a = (row[0] + row[1]) + (row[2] + row[3]);
+    a += dpp_f64<0xB1>(a);    // quad_perm 1,0,3,2
+    a += dpp_f64<0x4E>(a);    // quad_perm 2,3,0,1
+    a += dpp_f64<0x141>(a);   // row_half_mirror
+    a += dpp_f64<0x140>(a);   // row_mirror
+    double w4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w4[q] = readlane_f64(a, 16 * q);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[q][wid] = w4[q];
+    }
+    __syncthreads();
+    if (wid == 0) {
+        // lane 4 q + w holds wave w's quantity q
+        double b = lane < 4 * NW ? red[lane >> 2][lane & 3] : 0.0;
+        b += dpp_f64<0xB1>(b);
+        b += dpp_f64<0x4E>(b);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = readlane_f64(b, 4 * q);
+    }
+}
+
 template <int LAW>
 __device__ __forceinline__ void pair_mixed(const SysDev& S, float dx, float dy, float dz, float r2,
                                            float inv_s2, float ur, float al, float be,
@@ -791,7 +835,7 @@ __global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(St
     if (lc < S.ncell && (!KICK || S.active)) {
         const int c = S.cell_off + lc;
         int cx, cy, cz;
-        cell_coords(lc, S.nc, cx, cy, cz);
+        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         const int h0 = p.start[c], h1 = p.start[c + 1];
         float3* sf = st_f + ((PREC == MIXED) ? wid * PAD : 0);
         double2* sx = st_x + ((PREC == FP64) ? wid * PAD : 0);
@@ -1160,7 +1204,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     if (lc < S.ncell && (!KICK || S.active)) {
         const int c = S.cell_off + lc;
         int cx, cy, cz;
-        cell_coords(lc, S.nc, cx, cy, cz);
+        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
         const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
         float4* sw = st[wid];
@@ -1181,11 +1225,11 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
             cntq = p.start[nb + 1] - j0;
-            shx = (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]);
-            shy = (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]);
-            shz = (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]);
+            shx = S.shf[0][3 * (ox + 1) + ix + 1];
+            shy = S.shf[1][3 * (oy + 1) + iy + 1];
+            shz = S.shf[2][3 * (oz + 1) + iz + 1];
         }
-        const int incl = wave_incl_scan(cntq);
+        const int incl = wave_incl_scan_dpp(cntq);
         if (lane < 27) {
             tb_off[wid][lane] = incl - cntq;
             tb_delta[wid][lane] = j0 - (incl - cntq);
@@ -1422,7 +1466,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
         }
     }
     double out[4];
-    block_sum4(wacc, out);
+    block_sum4_lds(wacc, reinterpret_cast<double*>(st[wid]), out);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
@@ -1570,7 +1614,7 @@ __global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bin
     const int s = b * kBlock + threadIdx.x;
     if (s - S.atom_off < S.n && S.law != LAW_NONE) {
         int c[3];
-        cell_coords(p.scell[s] - S.cell_off, S.nc, c[0], c[1], c[2]);
+        cell_xyz_of(p.cell_xyz, p.scell[s], c[0], c[1], c[2]);
         int R[3];
         for (int a = 0; a < 3; ++a) R[a] = (int)ceil(rmax / S.a[a]);
         const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
