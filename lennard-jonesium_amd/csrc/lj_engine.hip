@@ -318,7 +318,7 @@ void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
     if constexpr (PREC == ljk::MIXED)
         launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
     else
-        launch(e, ljk::k_pair_cells<PREC, LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
+        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
 }
 
 template <int LAW, bool KICK>

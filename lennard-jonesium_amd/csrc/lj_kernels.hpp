@@ -712,27 +712,6 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 }
 
 
-// ---------------------------------------------------------------------------
-// 6b. tiled pair kernel: one wave64 per home cell.
-//
-//   * the home cell's n_i particles (chunks of <= 64) are spread over the wave
-//     as S = 64 / n_i "slices" of n_i lanes; lane (i, slice) owns particle i
-//     and every S-th staged neighbour starting at its slice;
-//   * the 27 neighbour cells (reference lattice-image rule) are staged in LDS
-//     in coalesced reads, pre-shifted by their image (mixed) or with the
-//     image shift kept beside the coordinates (FP64, so that r^2 is formed
-//     exactly as the reference forms it);
-//   * phase 1 tests at most 64 staged candidates per lane and records the
-//     in-cutoff ones in a 64-bit mask held in registers; phase 2 walks the set
-//     bits, so the expensive pair law runs only for in-cutoff pairs and the
-//     wave's divergence is bounded by the busiest lane;
-//   * slice partial forces are summed in a fixed order through LDS; the
-//     owning lane writes F, applies the second half kick and accumulates
-//     |v|^2 and |d|^2, and the block sums go to the cell-space partials.
-// ---------------------------------------------------------------------------
-constexpr int kStageMixed = 256;   // staged neighbours per wave (float4 each)
-constexpr int kStageFp64 = 128;    // staged neighbours per wave (x,sx,y,sy,z,sz)
-
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -779,347 +758,8 @@ __device__ __forceinline__ void block_sum4_lds(const double v[4], double* sc, do
     }
 }
 
-template <int LAW>
-__device__ __forceinline__ void pair_mixed(const SysDev& S, float dx, float dy, float dz, float r2,
-                                           float inv_s2, float ur, float al, float be,
-                                           float& ax, float& ay, float& az, double& pe, double& w)
-{
-    if (LAW == LAW_LJ) {
-        const float inv = __builtin_amdgcn_rcpf(r2);
-        const float t = inv * inv_s2;
-        const float s6 = t * t * t;
-        const float u = r2 * ur;
-        const float vv = __builtin_fmaf(4.0f * s6, s6 - 1.0f, __builtin_fmaf(be, u - 1.0f, al));
-        const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f, -2.0f * be * u);
-        const float fm = ww * inv;
-        ax = __builtin_fmaf(fm, dx, ax);
-        ay = __builtin_fmaf(fm, dy, ay);
-        az = __builtin_fmaf(fm, dz, az);
-        pe += (double)vv;
-        w += (double)ww;
-    } else {
-        const float nrm = sqrtf(r2);
-        const float vv = (float)S.strength * ((float)S.rc - nrm);
-        const float ww = (float)S.strength * nrm;
-        const float fm = ww / r2;
-        ax += fm * dx; ay += fm * dy; az += fm * dz;
-        pe += (double)vv;
-        w += (double)ww;
-    }
-}
-
-template <int PREC, int LAW, bool KICK>
-__global__ __launch_bounds__(kBlock, PREC == MIXED ? 6 : 3) void k_pair_cells(StepArgs p)
-{
-    constexpr int CAP = (PREC == MIXED) ? kStageMixed : kStageFp64;
-    constexpr int PAD = CAP + 64;    // room for the dummy tail read by the last round
-    constexpr int NW = kBlock / 64;
-    // per-wave staging buffers
-    __shared__ float3 st_f[(PREC == MIXED) ? NW * PAD : 1];
-    __shared__ double2 st_x[(PREC == FP64) ? NW * PAD : 1];
-    __shared__ double2 st_y[(PREC == FP64) ? NW * PAD : 1];
-    __shared__ double2 st_z[(PREC == FP64) ? NW * PAD : 1];
-    __shared__ double red[(PREC == FP64) ? NW : 1][64][4];
-    __shared__ float redf[(PREC == MIXED) ? NW : 1][64][4];
-    __shared__ int tb_start[NW][27], tb_off[NW][27];
-    __shared__ float4 tb_shf[(PREC == MIXED) ? NW : 1][27];
-    __shared__ double tb_shd[(PREC == FP64) ? NW : 1][27][3];
-
-    const int b = xcd_slab_block();
-    const int k = p.cblk_sys[b];
-    const SysDev& S = p.sys[k];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
-    double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
-
-    if (lc < S.ncell && (!KICK || S.active)) {
-        const int c = S.cell_off + lc;
-        int cx, cy, cz;
-        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
-        const int h0 = p.start[c], h1 = p.start[c + 1];
-        float3* sf = st_f + ((PREC == MIXED) ? wid * PAD : 0);
-        double2* sx = st_x + ((PREC == FP64) ? wid * PAD : 0);
-        double2* sy = st_y + ((PREC == FP64) ? wid * PAD : 0);
-        double2* sz = st_z + ((PREC == FP64) ? wid * PAD : 0);
-        const float rc2f = (float)S.rc2;
-        const double rc2d = S.rc2;
-        const float inv_s2 = (float)(1.0 / S.sigma2);
-        const bool unit_sigma = S.sigma2 == 1.0;
-        const float cv = (float)(S.beta * S.inv_rc2s);          // beta / rc^2 (reduced)
-        const float amb = (float)(S.alpha - S.beta);
-        const float cw = (float)(-2.0 * S.beta * S.inv_rc2s);
-
-        // ---- the 27 neighbour cells (reference lattice-image rule), one lane
-        //      each: slot range, image shift, and offsets in the concatenation
-        int cntq = 0;
-        if (lane < 27) {
-            const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
-            int nbx, ix, nby, iy, nbz, iz;
-            nb_axis(cx, ox, S.nc[0], nbx, ix);
-            nb_axis(cy, oy, S.nc[1], nby, iy);
-            nb_axis(cz, oz, S.nc[2], nbz, iz);
-            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
-            const int j0 = p.start[nb];
-            cntq = p.start[nb + 1] - j0;
-            tb_start[wid][lane] = j0;
-            if (PREC == MIXED)
-                tb_shf[wid][lane] = make_float4(
-                    (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]),
-                    (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]),
-                    (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]), 0.f);
-            else {
-                tb_shd[wid][lane][0] = ix * S.L[0];
-                tb_shd[wid][lane][1] = iy * S.L[1];
-                tb_shd[wid][lane][2] = iz * S.L[2];
-            }
-        }
-        const int incl = wave_incl_scan(cntq);
-        if (lane < 27) tb_off[wid][lane] = incl - cntq;
-        const int total = __shfl(incl, 26, 64);
-        wave_lds_sync();
-
-        for (int i0 = h0; i0 < h1; i0 += 64) {
-            const int ni = min(64, h1 - i0);
-            const int nsl = 64 / ni;
-            const bool act = lane < nsl * ni;
-            const int il = act ? lane % ni : 0;
-            const int sl = act ? lane / ni : 0;
-            const int si = i0 + il;
-            // the lane's own particle
-            double xi = 0, yi = 0, zi = 0;
-            float4 li = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (PREC == FP64) { xi = p.pos.x[si]; yi = p.pos.y[si]; zi = p.pos.z[si]; }
-            else li = p.lpos[si];
-            double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
-            float ax = 0.f, ay = 0.f, az = 0.f;
-            int nct = 0;
-            const int cap = min(CAP, 64 * nsl);
-
-            // bounding box of this chunk's particles; a neighbour farther than
-            // rc from it cannot interact with any of them
-            double blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
-            float flo[3] = {0.f, 0.f, 0.f}, fhi[3] = {0.f, 0.f, 0.f};
-            if (PREC == MIXED) {
-                const float c3[3] = {li.x, li.y, li.z};
-                for (int ax3 = 0; ax3 < 3; ++ax3) {
-                    float lo = act ? c3[ax3] : 3.0e38f, hi = act ? c3[ax3] : -3.0e38f;
-                    for (int o = 32; o > 0; o >>= 1) {
-                        lo = fminf(lo, __shfl_xor(lo, o, 64));
-                        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
-                    }
-                    flo[ax3] = lo;
-                    fhi[ax3] = hi;
-                }
-            } else {
-                const double c3[3] = {xi, yi, zi};
-                for (int ax3 = 0; ax3 < 3; ++ax3) {
-                    double lo = act ? c3[ax3] : 1e300, hi = act ? c3[ax3] : -1e300;
-                    for (int o = 32; o > 0; o >>= 1) {
-                        lo = fmin(lo, __shfl_xor(lo, o, 64));
-                        hi = fmax(hi, __shfl_xor(hi, o, 64));
-                    }
-                    blo[ax3] = lo;
-                    bhi[ax3] = hi;
-                }
-            }
-            const double keep2 = S.rc2 * (1.0 + 1e-6) + 1e-9;   // conservative margins
-            const float keepf = (float)keep2;
-            const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-
-            int g0 = 0;
-            while (g0 < total) {
-                // ---- stage the surviving neighbours of the next candidates, 64 at a time
-                int m = 0;
-                while (g0 < total && m <= cap - 64) {
-                    const int g = g0 + lane;
-                    bool keep = false;
-                    float3 pf = make_float3(0.f, 0.f, 0.f);
-                    double2 X = make_double2(0, 0), Y = X, Z = X;
-                    if (g < total) {
-                        int lo_q = 0, hi_q = 26;       // last cell q with off[q] <= g
-                        while (lo_q < hi_q) {
-                            const int mid = (lo_q + hi_q + 1) >> 1;
-                            if (tb_off[wid][mid] <= g) lo_q = mid; else hi_q = mid - 1;
-                        }
-                        const int j = tb_start[wid][lo_q] + (g - tb_off[wid][lo_q]);
-                        if (PREC == MIXED) {
-                            const float4 lj = p.lpos[j];
-                            const float4 sh = tb_shf[wid][lo_q];
-                            pf = make_float3(lj.x - sh.x, lj.y - sh.y, lj.z - sh.z);
-                            const float ox = fmaxf(fmaxf(flo[0] - pf.x, pf.x - fhi[0]), 0.f);
-                            const float oy = fmaxf(fmaxf(flo[1] - pf.y, pf.y - fhi[1]), 0.f);
-                            const float oz = fmaxf(fmaxf(flo[2] - pf.z, pf.z - fhi[2]), 0.f);
-                            keep = __builtin_fmaf(oz, oz, __builtin_fmaf(oy, oy, ox * ox)) < keepf;
-                        } else {
-                            X = make_double2(p.pos.x[j], tb_shd[wid][lo_q][0]);
-                            Y = make_double2(p.pos.y[j], tb_shd[wid][lo_q][1]);
-                            Z = make_double2(p.pos.z[j], tb_shd[wid][lo_q][2]);
-                            const double q3[3] = {X.x + X.y, Y.x + Y.y, Z.x + Z.y};   // image position
-                            double d2 = 0;
-                            for (int ax3 = 0; ax3 < 3; ++ax3) {
-                                const double o = fmax(fmax(blo[ax3] - q3[ax3], q3[ax3] - bhi[ax3]), 0.0);
-                                d2 += o * o;
-                            }
-                            keep = d2 < keep2;
-                        }
-                    }
-                    const unsigned long long bal = __ballot(keep);
-                    if (keep) {
-                        const int t = m + __popcll(bal & below);
-                        if (PREC == MIXED) sf[t] = pf;
-                        else { sx[t] = X; sy[t] = Y; sz[t] = Z; }
-                    }
-                    m += __popcll(bal);
-                    g0 += 64;
-                }
-                // dummy tail: far away, never inside the cutoff
-                if (lane < nsl) {
-                    if (PREC == MIXED) sf[m + lane] = make_float3(1e18f, 1e18f, 1e18f);
-                    else {
-                        sx[m + lane] = make_double2(1e150, 0.0);
-                        sy[m + lane] = make_double2(1e150, 0.0);
-                        sz[m + lane] = make_double2(1e150, 0.0);
-                    }
-                }
-                wave_lds_sync();
-                // ---- phase 1: cutoff test of <= 64 candidates per lane (uniform trip count)
-                const int kmax = (m + nsl - 1) / nsl;
-                unsigned lo = 0u, hi = 0u;
-                auto test = [&](int kb) -> bool {
-                    const int q = sl + kb * nsl;
-                    if (PREC == MIXED) {
-                        const float3 d = sf[q];
-                        const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
-                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        return r2 < rc2f;
-                    } else {
-                        const double2 X = sx[q], Y = sy[q], Z = sz[q];
-                        const double dx = (xi - X.x) - X.y;
-                        const double dy = (yi - Y.x) - Y.y;
-                        const double dz = (zi - Z.x) - Z.y;
-                        const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        return r2 < rc2d;
-                    }
-                };
-                const int k1 = min(kmax, 32);
-#pragma unroll 4
-                for (int kb = 0; kb < k1; ++kb) lo |= (test(kb) ? 1u : 0u) << kb;
-#pragma unroll 4
-                for (int kb = 32; kb < kmax; ++kb) hi |= (test(kb) ? 1u : 0u) << (kb - 32);
-                unsigned long long mask = act ? (((unsigned long long)hi << 32) | lo) : 0ull;
-                nct += __popcll(mask);
-                // ---- phase 2: pair law on the in-cutoff candidates only
-                float spe = 0.f, sw = 0.f;
-                while (mask) {
-                    const int kb = __builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const int q = sl + kb * nsl;
-                    if (PREC == MIXED) {
-                        const float3 d = sf[q];
-                        const float dx = li.x - d.x, dy = li.y - d.y, dz = li.z - d.z;
-                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        if (r2 == 0.f) { --nct; continue; }   // the particle itself
-                        if (LAW == LAW_LJ) {
-                            // s6 = (sigma/r)^6; V = 4 s6^2 - 4 s6 + (alpha - beta) + beta u,
-                            // W = 48 s6^2 - 24 s6 - 2 beta u, u = (r / rc)^2
-                            const float inv = __builtin_amdgcn_rcpf(r2);
-                            const float t = unit_sigma ? inv : inv * inv_s2;
-                            const float s6 = t * t * t;
-                            const float s12 = s6 * s6;
-                            spe += __builtin_fmaf(4.0f, s12, __builtin_fmaf(-4.0f, s6, __builtin_fmaf(r2, cv, amb)));
-                            const float ww = __builtin_fmaf(48.0f, s12, __builtin_fmaf(-24.0f, s6, r2 * cw));
-                            sw += ww;
-                            const float fm = ww * inv;
-                            ax = __builtin_fmaf(fm, dx, ax);
-                            ay = __builtin_fmaf(fm, dy, ay);
-                            az = __builtin_fmaf(fm, dz, az);
-                        } else {
-                            const float nrm = sqrtf(r2);
-                            const float ww = (float)S.strength * nrm;
-                            spe += (float)S.strength * ((float)S.rc - nrm);
-                            sw += ww;
-                            const float fm = ww / r2;
-                            ax += fm * dx; ay += fm * dy; az += fm * dz;
-                        }
-                    } else {
-                        const double2 X = sx[q], Y = sy[q], Z = sz[q];
-                        const double dx = (xi - X.x) - X.y;
-                        const double dy = (yi - Y.x) - Y.y;
-                        const double dz = (zi - Z.x) - Z.y;
-                        const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        if (r2 == 0.0) { --nct; continue; }   // the particle itself
-                        pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
-                    }
-                }
-                // per-stage float partials (<= 64 pairs) are folded into double sums
-                pe += (double)spe;
-                w += (double)sw;
-                wave_lds_sync();
-            }
-            if (PREC == MIXED && LAW == LAW_LJ && S.eps != 1.0) {
-                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
-                pe *= S.eps; w *= S.eps;
-            }
-            // ---- sum the slices of each particle in a fixed order
-            if (PREC == MIXED) {
-                redf[wid][lane][0] = ax;
-                redf[wid][lane][1] = ay;
-                redf[wid][lane][2] = az;
-                redf[wid][lane][3] = __int_as_float(nct);
-            } else {
-                red[wid][lane][0] = fx;
-                red[wid][lane][1] = fy;
-                red[wid][lane][2] = fz;
-                red[wid][lane][3] = (double)nct;
-            }
-            wave_lds_sync();
-            if (act && sl == 0) {
-                for (int t = 1; t < nsl; ++t) {
-                    if (PREC == MIXED) {
-                        ax += redf[wid][il + t * ni][0];
-                        ay += redf[wid][il + t * ni][1];
-                        az += redf[wid][il + t * ni][2];
-                        nct += __float_as_int(redf[wid][il + t * ni][3]);
-                    } else {
-                        fx += red[wid][il + t * ni][0];
-                        fy += red[wid][il + t * ni][1];
-                        fz += red[wid][il + t * ni][2];
-                        nct += (int)red[wid][il + t * ni][3];
-                    }
-                }
-                if (PREC == MIXED) { fx = ax; fy = ay; fz = az; }
-                if (p.ncount) p.ncount[si] = nct;
-                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
-                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-                if (KICK) {
-                    const double dt = S.dt;
-                    vx = vx + ((1. / 2.) * fx) * dt;
-                    vy = vy + ((1. / 2.) * fy) * dt;
-                    vz = vz + ((1. / 2.) * fz) * dt;
-                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
-                }
-                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
-                wacc[0] += (vx * vx + vy * vy) + vz * vz;
-                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
-            }
-            if (act) {
-                wacc[1] += (1. / 2.) * pe;
-                wacc[2] += (1. / 2.) * w;
-            }
-            wave_lds_sync();
-        }
-    }
-    double out[4];
-    block_sum4(wacc, out);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
-        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
-    }
-}
-
 // ---------------------------------------------------------------------------
-// 6c. tiled pair kernel, mixed precision: one wave64 per home cell, with the
+// 6b. tiled pair kernel, mixed precision: one wave64 per home cell, with the
 //     cutoff test reduced to three FMAs per candidate.
 //
 //   * coordinates are float, relative to the centre o of the chunk's bounding
@@ -1467,6 +1107,298 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     }
     double out[4];
     block_sum4_lds(wacc, reinterpret_cast<double*>(st[wid]), out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6c. tiled pair kernel, double precision: the candidate search of 6b on
+//     float copies, widened by a margin so that it keeps every pair, then the
+//     reference's own double test and pair law for the survivors.
+//
+//   * staged entry: float4 (image position relative to the chunk centre,
+//     |r|^2), the double position r_j and the index q of its cell in the
+//     27-cell table (which holds the image shift i L);
+//   * phase 1: |x - r|^2 < rc^2 + 1e-3 in float, a superset of the pairs
+//     (float rounding stays below 1e-4 here);
+//   * phase 2: sep = (r_i - r_j) - i L and r^2 = (x^2 + y^2) + z^2 exactly as
+//     the reference forms them, kept if 0 < r^2 < rc^2, then the law; the
+//     decisions are the reference's bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kRowBudgetD = 200;        // staged candidates per stage (at most)
+constexpr int kRowsD = kRowBudgetD + 4;  // + the rows read past kmax in phase 1
+
+constexpr SliceGeom slice_geom_d(int ni)
+{
+    const int nsl = 64 / ni;
+    int rows = kRowBudgetD / nsl;
+    if (rows > 64) rows = 64;
+    if (!(rows & 1) && nsl > 1 && nsl * (rows - 1) >= 128) --rows;   // odd where it fits
+    return SliceGeom{nsl, rows, (1u << 20) / (unsigned)nsl + 1u, nsl * rows,
+                     (1u << 20) / (unsigned)ni + 1u};
+}
+
+struct SliceGeomTableD {
+    SliceGeom g[65];
+    constexpr SliceGeomTableD() : g{}
+    {
+        for (int ni = 1; ni <= 64; ++ni) g[ni] = slice_geom_d(ni);
+    }
+};
+
+__constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
+
+template <int LAW, bool KICK>
+__global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ float4 st[NW][kRowsD];
+    __shared__ double sd[NW][kRowsD * 3];      // r_j of each entry; slice and block sums
+    __shared__ unsigned char sq[NW][kRowsD];   // table index of the entry's cell
+    __shared__ int tb_delta[NW][27];
+    __shared__ int tb_off[NW][32];             // entries 27..31 = INT_MAX (search padding)
+    __shared__ double tb_shd[NW][27][3];       // image shift i L of each neighbour cell
+    __shared__ double tb_rel[NW][27][3];       // i L - o (o: chunk centre)
+
+    const int b = xcd_slab_block();
+    const int k = p.cblk_sys[b];
+    const SysDev& S = p.sys[k];
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
+    double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
+    double* sdw = sd[wid];
+
+    if (lc < S.ncell && (!KICK || S.active)) {
+        const int c = S.cell_off + lc;
+        int cx, cy, cz;
+        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+        const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
+        const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
+        float4* sw = st[wid];
+        const double rc2d = S.rc2;
+        const float rc2f = S.rc2f;
+        const float keepf = rc2f + 2e-3f;   // bounding-box pruning, wider than the phase-1 margin
+
+        // ---- the 27 neighbour cells (reference lattice-image rule), one lane each
+        int cntq = 0, j0 = 0;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (lane < 27) {
+            const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+            int nbx, ix, nby, iy, nbz, iz;
+            nb_axis(cx, ox, S.nc[0], nbx, ix);
+            nb_axis(cy, oy, S.nc[1], nby, iy);
+            nb_axis(cz, oz, S.nc[2], nbz, iz);
+            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+            j0 = p.start[nb];
+            cntq = p.start[nb + 1] - j0;
+            sx = ix * S.L[0];
+            sy = iy * S.L[1];
+            sz = iz * S.L[2];
+            tb_shd[wid][lane][0] = sx;
+            tb_shd[wid][lane][1] = sy;
+            tb_shd[wid][lane][2] = sz;
+        }
+        const int incl = wave_incl_scan_dpp(cntq);
+        if (lane < 27) {
+            tb_off[wid][lane] = incl - cntq;
+            tb_delta[wid][lane] = j0 - (incl - cntq);
+        } else if (lane < 32) {
+            tb_off[wid][lane] = 0x7fffffff;
+        }
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 26, 64));
+        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+        for (int i0 = h0; i0 < h1; i0 += 64) {
+            const int ni = min(64, h1 - i0);
+            const SliceGeom G = kSliceGeomD.g[ni];
+            const int nsl = G.nsl;
+            const bool act = lane < nsl * ni;
+            const int q_ni = (int)(((unsigned)lane * G.magic_ni) >> 20);   // lane / ni
+            const int il = act ? lane - q_ni * ni : 0;
+            const int sl = act ? q_ni : 0;
+            const int si = i0 + il;
+            const double xi = p.pos.x[si], yi = p.pos.y[si], zi = p.pos.z[si];
+
+            // bounding box of the chunk in float (positions are >= 0 after the
+            // wrap, so their float bit patterns order as signed ints)
+            float o3[3], h3[3];
+            {
+                const float c3[3] = {(float)xi, (float)yi, (float)zi};
+#pragma unroll
+                for (int a3 = 0; a3 < 3; ++a3) {
+                    const int bits = __float_as_int(c3[a3]);
+                    const float lo = __int_as_float(wave_extreme<false>(act ? bits : 0x7fffffff));
+                    const float hi = __int_as_float(wave_extreme<true>(act ? bits : (int)0x80000000));
+                    o3[a3] = 0.5f * (lo + hi);
+                    h3[a3] = 0.5f * (hi - lo) + 1e-4f * (hi + 1.0f);   // + float rounding of r_i
+                }
+            }
+            const float xf = (float)(xi - (double)o3[0]);
+            const float yf = (float)(yi - (double)o3[1]);
+            const float zf = (float)(zi - (double)o3[2]);
+            const float m2x = -2.0f * xf, m2y = -2.0f * yf, m2z = -2.0f * zf;
+            const float thr = (rc2f + 1e-3f) - __builtin_fmaf(zf, zf, __builtin_fmaf(yf, yf, xf * xf));
+
+            const int cap = G.cap;
+            const int SL = G.rows;
+            const unsigned magic = G.magic;   // t / nsl = t * magic >> 20
+            const float4* rowp = sw + sl * SL;
+
+            wave_lds_sync();   // the previous chunk's reads of tb_rel / st / sd are done
+            if (lane < 27) {
+                tb_rel[wid][lane][0] = sx - (double)o3[0];
+                tb_rel[wid][lane][1] = sy - (double)o3[1];
+                tb_rel[wid][lane][2] = sz - (double)o3[2];
+            }
+            wave_lds_sync();
+
+            // candidate g (clamped to the last one, so every lane loads): its
+            // table index q and double position, fetched one round ahead
+            auto fetch = [&](int g, int& q, double& x, double& y, double& z) {
+                const int gc = min(g, total - 1);
+                int pos = 0;
+#pragma unroll
+                for (int step = 16; step > 0; step >>= 1)
+                    if (tb_off[wid][pos + step] <= gc) pos += step;
+                q = pos;
+                const unsigned j = (unsigned)(tb_delta[wid][pos] + gc);
+                x = p.pos.x[j];
+                y = p.pos.y[j];
+                z = p.pos.z[j];
+            };
+            int nq;
+            double nx, ny, nz;
+            fetch(lane, nq, nx, ny, nz);
+            int g0 = 0;
+            double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, w = 0.0;
+            int nct = 0;
+            while (g0 < total) {
+                // ---- stage the surviving neighbours of the next candidates, 64 at a time
+                int m = 0;
+                while (g0 < total && m <= cap - 64) {
+                    const int cq = nq;
+                    const double cxd = nx, cyd = ny, czd = nz;
+                    const int g = g0 + lane;
+                    g0 += 64;
+                    fetch(g0 + lane, nq, nx, ny, nz);
+                    float4 rv;
+                    rv.x = (float)(cxd + tb_rel[wid][cq][0]);
+                    rv.y = (float)(cyd + tb_rel[wid][cq][1]);
+                    rv.z = (float)(czd + tb_rel[wid][cq][2]);
+                    const float ex = fmaxf(fabsf(rv.x) - h3[0], 0.f);
+                    const float ey = fmaxf(fabsf(rv.y) - h3[1], 0.f);
+                    const float ez = fmaxf(fabsf(rv.z) - h3[2], 0.f);
+                    const bool keep =
+                        g < total && __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < keepf;
+                    rv.w = __builtin_fmaf(rv.z, rv.z, __builtin_fmaf(rv.y, rv.y, rv.x * rv.x));
+                    const unsigned long long bal = __ballot(keep);
+                    if (keep) {
+                        const unsigned t = (unsigned)m + (unsigned)__popcll(bal & below);
+                        const unsigned row = (t * magic) >> 20;
+                        const unsigned e = (t - row * (unsigned)nsl) * (unsigned)SL + row;
+                        sw[e] = rv;
+                        sdw[3 * e] = cxd;
+                        sdw[3 * e + 1] = cyd;
+                        sdw[3 * e + 2] = czd;
+                        sq[wid][e] = (unsigned char)cq;
+                    }
+                    m += __popcll(bal);
+                }
+                const int kmax = __builtin_amdgcn_readfirstlane(
+                    (int)(((unsigned)(m + nsl - 1) * magic) >> 20));
+                // far-away fillers for the unused tail of the last row
+                if (lane < kmax * nsl - m) {
+                    const int t = m + lane;
+                    const int row = (int)(((unsigned)t * magic) >> 20);
+                    sw[(t - row * nsl) * SL + row] = make_float4(1e18f, 1e18f, 1e18f, 3e36f);
+                }
+                wave_lds_sync();
+                if (kmax == 0) continue;
+
+                // ---- phase 1: kmax rows per lane, four per step (rows past kmax are dropped)
+                const int k4 = (kmax + 3) & ~3;
+                const int nlo = min(k4, 32), nhi = k4 - nlo;
+                unsigned lo = 0u, hi = 0u;
+                for (int kb = 0; kb < nlo; kb += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float4 d = rowp[kb + u];
+                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
+                        lo = __builtin_amdgcn_alignbit(lo, (unsigned)__float_as_int(v - thr), 31);
+                    }
+                }
+                for (int kb = 32; kb < 32 + nhi; kb += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float4 d = rowp[kb + u];
+                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
+                        hi = __builtin_amdgcn_alignbit(hi, (unsigned)__float_as_int(v - thr), 31);
+                    }
+                }
+                // row kb -> bit 63 - kb
+                lo = (nlo == 32) ? lo : (lo << (32 - nlo));
+                hi = (nhi == 0) ? 0u : ((nhi == 32) ? hi : (hi << (32 - nhi)));
+                unsigned long long mask = (((unsigned long long)lo << 32) | hi) & (~0ull << (64 - kmax));
+                if (!act) mask = 0ull;
+
+                // ---- phase 2: the reference's test and law on every candidate that passed
+                const int etop = sl * SL + 63;
+                while (mask) {
+                    const int bit = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const int e = etop - bit;
+                    const int q = sq[wid][e];
+                    const double dx = (xi - sdw[3 * e]) - tb_shd[wid][q][0];
+                    const double dy = (yi - sdw[3 * e + 1]) - tb_shd[wid][q][1];
+                    const double dz = (zi - sdw[3 * e + 2]) - tb_shd[wid][q][2];
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 < rc2d && r2 > 0.0) {
+                        pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+                        ++nct;
+                    }
+                }
+                wave_lds_sync();
+            }
+            // ---- sum the slices of each particle in a fixed order
+            sdw[4 * lane] = fx;
+            sdw[4 * lane + 1] = fy;
+            sdw[4 * lane + 2] = fz;
+            sdw[4 * lane + 3] = (double)nct;
+            wave_lds_sync();
+            if (act && sl == 0) {
+                for (int t = 1; t < nsl; ++t) {
+                    const int o = 4 * (il + t * ni);
+                    fx += sdw[o];
+                    fy += sdw[o + 1];
+                    fz += sdw[o + 2];
+                    nct += (int)sdw[o + 3];
+                }
+                if (p.ncount) p.ncount[si] = nct;
+                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
+                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+                if (KICK) {
+                    const double dt = S.dt;
+                    vx = vx + ((1. / 2.) * fx) * dt;
+                    vy = vy + ((1. / 2.) * fy) * dt;
+                    vz = vz + ((1. / 2.) * fz) * dt;
+                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
+                }
+                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                wacc[0] += (vx * vx + vy * vy) + vz * vz;
+                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
+            }
+            if (act) {
+                wacc[1] += (1. / 2.) * pe;
+                wacc[2] += (1. / 2.) * w;
+            }
+        }
+    }
+    double out[4];
+    block_sum4_lds(wacc, sdw, out);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
