@@ -110,6 +110,7 @@ struct lj_engine {
     hipStream_t stream = nullptr;
     bool use_graphs = true;
     hipGraphExec_t graph2 = nullptr;   // two steps starting from buffer set 0
+    hipGraphExec_t graphN = nullptr;   // kGraphSteps steps starting from buffer set 0
     int graph_cap = -1;               // record capacity baked into the graph
 
     bool profiling = false;
@@ -395,20 +396,22 @@ int ensure_records(lj_engine* e, int nsteps)
     return LJ_OK;
 }
 
-int build_graph(lj_engine* e)
+// Steps per long graph.  Each graph launch ends with a release of the caches
+// that costs several microseconds once the pair kernel has left the L2 full of
+// dirty lines, so long runs replay many steps per launch.  Any even count
+// returns the state to buffer set 0.
+constexpr int kGraphSteps = 32;
+
+// Captures nsteps steps starting from buffer set 0 into *out.
+int capture_steps(lj_engine* e, int nsteps, hipGraphExec_t* out)
 {
-    if (e->graph2 && e->graph_cap == e->rec_cap) return LJ_OK;
-    if (e->graph2) {
-        HIP_TRY(hipGraphExecDestroy(e->graph2));
-        e->graph2 = nullptr;
-    }
     const int saved = e->cur;
     const bool saved_sorted = e->sorted;
     e->cur = 0;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(e);
-    if (!rc) rc = enqueue_step(e);
+    int rc = LJ_OK;
+    for (int s = 0; s < nsteps && !rc; ++s) rc = enqueue_step(e);
     hipError_t ce = hipStreamEndCapture(e->stream, &g);
     e->cur = saved;
     e->sorted = saved_sorted;
@@ -418,10 +421,30 @@ int build_graph(lj_engine* e)
     }
     if (ce != hipSuccess) return fail(LJ_E_HIP, std::string("hipStreamEndCapture: ") +
                                                     hipGetErrorString(ce));
-    hipError_t ie = hipGraphInstantiate(&e->graph2, g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess)
         return fail(LJ_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    return LJ_OK;
+}
+
+void drop_graphs(lj_engine* e)
+{
+    if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
+    if (e->graphN) (void)hipGraphExecDestroy(e->graphN);
+    e->graph2 = e->graphN = nullptr;
+}
+
+int build_graph(lj_engine* e)
+{
+    if (e->graph2 && e->graphN && e->graph_cap == e->rec_cap) return LJ_OK;
+    drop_graphs(e);
+    int rc = capture_steps(e, 2, &e->graph2);
+    if (!rc) rc = capture_steps(e, kGraphSteps, &e->graphN);
+    if (rc) {
+        drop_graphs(e);
+        return rc;
+    }
     e->graph_cap = e->rec_cap;
     return LJ_OK;
 }
@@ -662,7 +685,7 @@ int lj_engine_destroy(lj_engine* e)
 {
     if (!e) return LJ_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
+    drop_graphs(e);
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_perm_id,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
@@ -768,6 +791,11 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
         if (e->cur != 0) {
             if ((rc = enqueue_step(e))) return rc;
             ++done;
+        }
+        while (nsteps - done >= kGraphSteps) {
+            HIP_TRY(hipGraphLaunch(e->graphN, e->stream));
+            done += kGraphSteps;
+            e->sorted = true;
         }
         while (nsteps - done >= 2) {
             HIP_TRY(hipGraphLaunch(e->graph2, e->stream));
@@ -1026,9 +1054,8 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
     if (rc) return rc;
     if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED)
         return fail(LJ_E_INVALID, "unknown pair kernel variant");
-    if (variant != e->pair_variant && e->graph2) {
-        HIP_TRY(hipGraphExecDestroy(e->graph2));
-        e->graph2 = nullptr;
+    if (variant != e->pair_variant && (e->graph2 || e->graphN)) {
+        drop_graphs(e);
     }
     e->pair_variant = variant;
     return LJ_OK;

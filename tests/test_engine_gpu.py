@@ -394,3 +394,27 @@ def test_clumped_cell_chunks_and_stage_splits(kernel):
     near = (np.abs(np.sqrt((sep ** 2).sum(1)) - 2.5) < 1e-5).sum()
     assert np.abs(got - _oracle_counts(pos, 10.0)).sum() <= 2 * near + 4
     e.close()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_records_independent_of_call_boundaries(graphs):
+    """Each step's record is finished inside the next step (or at the end of
+    the call): two calls of 50 steps must reproduce one call of 100, bit for
+    bit, for every system and both graph sizes."""
+    eng = _engine_mod()
+    cases = [(500, 0.8, 0.7, 3), (864, 1.1, 0.9, 5)]
+    specs, states = [], []
+    for n, rho, T, seed in cases:
+        st, L = ljref.initial_condition(n, rho, T, seed=seed)
+        specs.append(eng.SystemSpec(n, L))
+        states.append(st)
+    runs = []
+    for split in ((100,), (50, 50)):
+        e = eng.Engine(specs, precision="mixed", graphs=graphs)
+        for k, st in enumerate(states):
+            e.set_state(k, st.pos, st.vel)
+        runs.append(np.concatenate([e.step(n) for n in split], axis=0))
+        e.close()
+    assert runs[0].shape == runs[1].shape == (100, 2, 7)
+    assert np.array_equal(runs[0], runs[1])
+    assert np.all(np.diff(runs[0][:, :, 0], axis=0) > 0)   # time advances every step
