@@ -173,7 +173,7 @@ def liquid(args, eng):
         "force_rms_rel_err_vs_fp64": frms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": traffic,
-                     "kernel": ("k_pair_cells<MIXED,LJ,KICK>" if args.pair_kernel == "tiled"
+                     "kernel": ("k_pair_cells_mixed<LJ,KICK>" if args.pair_kernel == "tiled"
                                 else "k_pair<MIXED,LJ,KICK>"),
                      "kernel_ms": ktm["pair"], "flops_per_launch": flops,
                      "c_half": c_half, "p_half": p_half},

@@ -61,6 +61,8 @@ struct DevBuf {
     }
 };
 
+constexpr int kMaxParts = 64;   // first-level parts per system of a large grid's record sums
+
 struct KernelTimer {
     const char* name;
     double total_ms = 0;
@@ -97,6 +99,7 @@ struct lj_engine {
     double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
     int* d_step = nullptr;
     int* d_pending = nullptr;   // per system: a step awaits its finalize
+    double* d_partial2 = nullptr;   // [nsys][kMaxParts][4] first-level record sums
     double* d_records = nullptr;
     int rec_cap = 0;
     double* d_measure = nullptr;
@@ -176,11 +179,22 @@ struct lj_engine {
         f.cell_blocks = last_pair_cells ? 1 : 0;
         f.pending = d_pending;
         if (mode == 0) f.cell_blocks = step_uses_cells() ? 1 : 0;
+        f.partial2 = d_partial2;
+        // first-level parts: about 1024 block sums each, at most kMaxParts
+        int nbmax = 1;
+        for (const SysDev& s : sys) nbmax = std::max(nbmax, f.cell_blocks ? s.ncblk : s.nblk);
+        f.nf = std::min(kMaxParts, (nbmax + 1023) / 1024);
         return f;
     }
 
     // A step's pair kernel is the tiled one (cell-space block sums) unless the
     // law has no pair force or the per-particle variant is selected.
+    // Grids above the single-block scan's reach sum records in two levels.
+    bool two_level_records() const
+    {
+        return has_cells && ncell_total > ljk::kScanOneMax * 4 * ljk::kScanOneThreads;
+    }
+
     bool step_uses_cells() const
     {
         return sys[0].law != ljk::LAW_NONE && pair_variant == LJ_PAIR_CELL_TILED;
@@ -271,7 +285,7 @@ int enqueue_cells(lj_engine* e, bool integrate)
             launch(e, ljk::k_integrate_key<0, true>, gb, tb, p);
         });
     if (rc) return rc;
-    if (e->ncell_total <= ljk::kScanOneMax * 4 * ljk::kScanOneThreads) {
+    if (!e->two_level_records()) {
         // a step also finishes the previous step's records (pending flags decide)
         const ljk::FinalArgs f = e->fargs(0);
         const int nfin = integrate ? e->nsys : 0;
@@ -280,13 +294,15 @@ int enqueue_cells(lj_engine* e, bool integrate)
              })))
             return rc;
     } else {
-        if (integrate && (rc = enqueue_finalize(e, 0))) return rc;
+        // the previous step's records in two levels inside the two scan launches
+        ljk::FinalArgs f = e->fargs(0);
+        if (!integrate) f.nsys = 0;
         if ((rc = timed(e, 1, "scan_tiles", [&] {
-                 launch(e, ljk::k_scan_tiles, dim3(e->ntile), tb, p);
+                 launch(e, ljk::k_scan_tiles, dim3(e->ntile + f.nsys * f.nf), tb, p, f);
              })))
             return rc;
         if ((rc = timed(e, 2, "scan_apply", [&] {
-                 launch(e, ljk::k_scan_apply, dim3(e->ntile), tb, p);
+                 launch(e, ljk::k_scan_apply, dim3(e->ntile + f.nsys), tb, p, f);
              })))
             return rc;
     }
@@ -360,7 +376,12 @@ int enqueue_finalize(lj_engine* e, int mode)
 {
     const ljk::FinalArgs f = e->fargs(mode);
     return timed(e, 6, "finalize", [&] {
-        launch(e, ljk::k_finalize, dim3(e->nsys), dim3(kBlock), f);
+        if (e->two_level_records()) {
+            launch(e, ljk::k_finalize_parts, dim3(e->nsys * f.nf), dim3(kBlock), f);
+            launch(e, ljk::k_finalize_reduced, dim3(e->nsys), dim3(kBlock), f);
+        } else {
+            launch(e, ljk::k_finalize, dim3(e->nsys), dim3(ljk::kFinalThreads), f);
+        }
     });
 }
 
@@ -620,6 +641,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
         (rc = dev_alloc(&e->d_pending, nsys)) ||
+        (rc = dev_alloc(&e->d_partial2, (size_t)nsys * kMaxParts * 4)) ||
         (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
         return cleanup_fail(rc);
     for (int b = 0; b < 2; ++b) {
@@ -689,7 +711,7 @@ int lj_engine_destroy(lj_engine* e)
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_perm_id,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
-                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_records,
+                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
                     e->d_measure, e->d_rdf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);

@@ -207,10 +207,15 @@ struct FinalArgs {
     int mode;
     int cell_blocks;   // partial sums come from cell-space blocks (tiled pair kernel)
     int* pending;      // per system: set by a step's pair kernel, cleared here
+    double* partial2;  // [nsys][nf] first-level sums of large grids (see k_scan_tiles)
+    int nf;            // first-level parts per system
 };
 
 template <int NT>
 __device__ void finalize_system(const FinalArgs& f, int k);
+template <int NT>
+__device__ void finalize_range(const FinalArgs& f, int k, const double* part, int nb);
+__device__ void finalize_part(const FinalArgs& f, int k, int r);
 
 // ---------------------------------------------------------------------------
 // 1. kick + drift + wrap + cell key + cell count
@@ -339,8 +344,17 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total)
     return off + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p)
+//    Blocks past the tiles compute the first level of the previous step's
+//    record sums (f.nsys * f.nf of them; section 7), k_scan_apply's extra
+//    blocks the second.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p, FinalArgs f)
 {
+    const int ntile = (p.ncell_total + kScanTile - 1) / kScanTile;
+    if ((int)blockIdx.x >= ntile) {
+        const int x = blockIdx.x - ntile;
+        finalize_part(f, x / f.nf, x % f.nf);
+        return;
+    }
     const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
     int v = 0;
 #pragma unroll
@@ -351,8 +365,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p)
     if (threadIdx.x == 0) p.tile_sum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p)
+__global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p, FinalArgs f)
 {
+    const int ntile = (p.ncell_total + kScanTile - 1) / kScanTile;
+    if ((int)blockIdx.x >= ntile) {
+        const int k = blockIdx.x - ntile;
+        finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
+        return;
+    }
     __shared__ int s_off;
     // offset of this tile: sum of the preceding tile sums
     int acc = 0;
@@ -1415,16 +1435,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
 // Reduces system k's block sums into its record (one NT-thread block).
 // Mode 0 runs only while the system's step is pending and clears the flag.
 template <int NT>
-__device__ void finalize_system(const FinalArgs& f, int k)
+__device__ void finalize_range(const FinalArgs& f, int k, const double* part, int nb_all)
 {
     const SysDev& S = f.sys[k];
     const int pend = f.mode == 0 ? f.pending[k] : 1;
     double acc[4] = {0, 0, 0, 0};
-    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
-    const int nb = pend ? (f.cell_blocks ? S.ncblk : S.nblk) : 0;
-#pragma unroll 8
+    const int nb = pend ? nb_all : 0;
+#pragma unroll 16
     for (int b = threadIdx.x; b < nb; b += NT) {
-        const double4 v = reinterpret_cast<const double4*>(f.partial)[b0 + b];
+        const double4 v = reinterpret_cast<const double4*>(part)[b];
         acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
     }
     double out[4];
@@ -1467,9 +1486,60 @@ __device__ void finalize_system(const FinalArgs& f, int k)
     r[6] = out[3] / (double)S.n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_finalize(FinalArgs f)
+// Reduces system k's block sums (pair-kernel partials) into its record.
+template <int NT>
+__device__ void finalize_system(const FinalArgs& f, int k)
 {
-    finalize_system<kBlock>(f, blockIdx.x);
+    const SysDev& S = f.sys[k];
+    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
+    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
+    finalize_range<NT>(f, k, f.partial + 4 * (size_t)b0, nb);
+}
+
+// First level for large grids: part r of f.nf contiguous parts of system k's
+// block sums, summed by one kBlock-thread block into f.partial2.
+__device__ void finalize_part(const FinalArgs& f, int k, int r)
+{
+    const SysDev& S = f.sys[k];
+    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
+    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
+    const int chunk = (nb + f.nf - 1) / f.nf;
+    const int lo = min(nb, r * chunk), hi = min(nb, lo + chunk);
+    double acc[4] = {0, 0, 0, 0};
+#pragma unroll 8
+    for (int b = lo + (int)threadIdx.x; b < hi; b += kBlock) {
+        const double4 v = reinterpret_cast<const double4*>(f.partial)[b0 + b];
+        acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
+    }
+    double out[4];
+    block_sum4<kBlock>(acc, out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f.partial2[4 * ((size_t)k * f.nf + r) + q] = out[q];
+    }
+}
+
+// One 1024-thread block per system (a large system has tens of thousands of
+// block sums: more threads keep more loads in flight).
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void k_finalize(FinalArgs f)
+{
+    finalize_system<kFinalThreads>(f, blockIdx.x);
+}
+
+// The two levels of k_scan_tiles / k_scan_apply as launches of their own, for
+// records finished outside a cell build on large grids (the same sums in the
+// same order, so a record does not depend on where a run was split).
+__global__ __launch_bounds__(kBlock) void k_finalize_parts(FinalArgs f)
+{
+    finalize_part(f, blockIdx.x / f.nf, blockIdx.x % f.nf);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize_reduced(FinalArgs f)
+{
+    const int k = blockIdx.x;
+    finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
 }
 
 __global__ void k_reset_steps(int* step, int nsys)

@@ -418,3 +418,23 @@ def test_records_independent_of_call_boundaries(graphs):
     assert runs[0].shape == runs[1].shape == (100, 2, 7)
     assert np.array_equal(runs[0], runs[1])
     assert np.all(np.diff(runs[0][:, :, 0], axis=0) > 0)   # time advances every step
+
+
+def test_large_cell_grid_records_track_oracle():
+    """More than 32768 cells (dilute box, 37^3 cells): the two-launch cell scan
+    and the two-level record sums; FP64 records follow the oracle, and split
+    calls reproduce a single call bit for bit."""
+    eng = _engine_mod()
+    st, L = ljref.initial_condition(40000, 0.05, 1.0, seed=11)
+    assert int(L // 2.5) ** 3 > 32768
+    ref = ljref.run(st.copy(), L, 0.005, 40)
+    runs = []
+    for split in ((40,), (13, 27)):
+        e = eng.Engine([eng.SystemSpec(40000, L)])
+        e.set_state(0, st.pos, st.vel)
+        runs.append(np.concatenate([e.step(n) for n in split], axis=0)[:, 0, :])
+        e.close()
+    assert np.array_equal(runs[0], runs[1])
+    err = np.abs(runs[0] - ref)
+    _say("large grid max abs err", err.max(), "max |ref|", np.abs(ref).max())
+    assert np.all(err <= 1e-10 * np.abs(ref) + 1e-12)   # PE and W are ~0 in this gas
