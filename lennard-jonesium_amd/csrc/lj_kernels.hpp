@@ -31,6 +31,7 @@ namespace ljk {
 constexpr int kBlock = 256;        // slots per block in particle-space kernels
 constexpr int kScanTile = 1024;    // cells per block in the scan kernels
 constexpr int kCellsPerBlock = 4;  // one wave64 per home cell in the tiled pair kernel
+constexpr int kHistBins = 1024;    // LDS histogram window of the cell count (cells)
 
 enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
 enum Prec { FP64 = 0, MIXED = 1 };
@@ -104,6 +105,21 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
     v += dpp_f64<0x141>(v);   // row_half_mirror
     v += dpp_f64<0x140>(v);   // row_mirror
     return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Wave-wide minimum (MAX = false) or maximum of an int: four in-row DPP steps
+// (quad swaps, half-row and row mirrors), then the two cross-row exchanges.
+template <bool MAX>
+__device__ __forceinline__ int wave_extreme(int v)
+{
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));    // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));    // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));   // row_half_mirror
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));   // row_mirror
+    v = op(v, __shfl_xor(v, 16, 64));
+    v = op(v, __shfl_xor(v, 32, 64));
+    return v;
 }
 
 // Sums four per-thread doubles over an NT-thread block; thread 0 gets them.
@@ -276,10 +292,44 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
         }
     }
     if (KEYS) {
-        // Slots arrive nearly cell-sorted (the previous step's order), so equal
-        // keys form runs across the wave: one atomic per run reserves the run's
-        // arrival ranks, handed to its members by position.
-        const int lane = threadIdx.x & 63;
+        // Slots arrive nearly cell-sorted (the previous step's order), so a
+        // block's keys fall in a narrow window of cells: count them in an LDS
+        // histogram (LDS atomics give each particle its rank inside the block),
+        // then reserve each occupied bin's ranks with one global atomic.
+        __shared__ int hist[kHistBins];
+        __shared__ int win[2][kBlock / 64];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        const int lo = wave_extreme<false>(c >= 0 ? c : 0x7fffffff);
+        const int hi = wave_extreme<true>(c);
+        if (lane == 0) { win[0][wid] = lo; win[1][wid] = hi; }
+        __syncthreads();
+        int kmin = 0x7fffffff, kmax = -1;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            kmin = min(kmin, win[0][w]);
+            kmax = max(kmax, win[1][w]);
+        }
+        if (kmax < 0) return;   // no particles in this block
+        const int width = kmax - kmin + 1;
+        if (width <= kHistBins) {
+            for (int t = threadIdx.x; t < width; t += kBlock) hist[t] = 0;
+            __syncthreads();
+            const int r = (c >= 0) ? atomicAdd(&hist[c - kmin], 1) : 0;
+            __syncthreads();
+            for (int t = threadIdx.x; t < width; t += kBlock) {
+                const int n = hist[t];
+                if (n) hist[t] = atomicAdd(&p.count[kmin + t], n);
+            }
+            __syncthreads();
+            if (c >= 0) {
+                p.cell_of[s] = c;
+                p.slot_of[s] = hist[c - kmin] + r;
+            }
+            return;
+        }
+        // Wide windows (slots not yet in cell order): one atomic per run of
+        // equal keys in the wave reserves the run's arrival ranks, handed to
+        // its members by position.
         const int prev = __shfl_up(c, 1, 64);
         const bool head = (lane == 0) || (c != prev);
         const unsigned long long heads = __ballot(head);
@@ -828,20 +878,6 @@ struct SliceGeomTable {
 
 __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 
-// Wave-wide minimum (MAX = false) or maximum of an int: four in-row DPP steps
-// (quad swaps, half-row and row mirrors), then the two cross-row exchanges.
-template <bool MAX>
-__device__ __forceinline__ int wave_extreme(int v)
-{
-    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));    // quad_perm 1,0,3,2
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));    // quad_perm 2,3,0,1
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));   // row_half_mirror
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));   // row_mirror
-    v = op(v, __shfl_xor(v, 16, 64));
-    v = op(v, __shfl_xor(v, 32, 64));
-    return v;
-}
 
 template <int LAW, bool KICK>
 __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
