@@ -207,7 +207,10 @@ def sweep(args, eng):
         for k, (p, v) in enumerate(states):
             e.set_state(k, p, v)
         variant, ms = e.autotune(10)
-        log(f"{prec}: autotuner chose {variant} ({ms:.3f} ms/step)")
+        kinds = e.pair_kernels()
+        n_tiled = sum(1 for x in kinds if x == "tiled")
+        log(f"{prec}: autotuner chose {variant} ({ms:.3f} ms/step; "
+            f"{n_tiled} replicas tiled, {len(kinds) - n_tiled} per-particle)")
         rec, t = timed_run(e, None, None, steps, args.warmup, states)
         rate = total * steps / t
         # drift relative to max(|E_0|, KE_0 + |PE_0|)
@@ -216,6 +219,7 @@ def sweep(args, eng):
                   for k in range(len(specs))]
         worst = int(np.argmax(drifts))
         out[prec] = dict(value=rate, seconds=t, pair_kernel=variant,
+                         replicas_tiled=n_tiled, replicas_per_particle=len(kinds) - n_tiled,
                          max_replica_drift=float(max(drifts)),
                          median_replica_drift=float(np.median(drifts)),
                          worst_replica={"T": meta[worst][0], "rho": meta[worst][1],

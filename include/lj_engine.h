@@ -199,16 +199,32 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
 
 /* Pair-kernel variant: LJ_PAIR_CELL_TILED (default) runs one wave64 per home
  * cell with the neighbour cells staged in LDS; LJ_PAIR_PER_PARTICLE runs one
- * thread per particle over the 27 neighbour cells.  Both give the same pair
- * set and, in FP64, the same per-pair arithmetic. */
+ * thread per particle over the 27 neighbour cells; LJ_PAIR_BY_OCCUPANCY picks
+ * per system: tiled where the mean cell occupancy N / cells is at least the
+ * threshold of lj_engine_set_pair_split (default 4), per-particle elsewhere
+ * (dilute replicas of a sweep), both kernels running in the same step.  All
+ * give the same pair set and, in FP64, the same per-pair arithmetic.  No
+ * counterpart in the reference (its single pair filter,
+ * engine/particle_pair_filter.cpp:131-192, has no variants). */
 #define LJ_PAIR_PER_PARTICLE 0
 #define LJ_PAIR_CELL_TILED 1
+#define LJ_PAIR_BY_OCCUPANCY 2
 int lj_engine_set_pair_kernel(lj_engine* e, int variant);
 
-/* Project autotuner: times `steps` steps of every pair-kernel variant on the
- * engine's own systems (from the current state, which is saved and restored,
- * so tuning leaves no trace), then keeps the fastest.  chosen / ms_per_step
- * (optional) receive the winner and its time per step. */
+/* Selects LJ_PAIR_BY_OCCUPANCY with the given threshold (particles per cell). */
+int lj_engine_set_pair_split(lj_engine* e, double min_occupancy);
+
+/* The kernel each system uses now: per_system[k] = LJ_PAIR_CELL_TILED or
+ * LJ_PAIR_PER_PARTICLE. */
+int lj_engine_pair_kernels(lj_engine* e, int* per_system);
+
+/* Project autotuner: times `steps` steps of each pair-kernel assignment --
+ * all tiled, all per-particle, and per-system splits at cell occupancies 1.5,
+ * 3, 6 and 12 that give distinct assignments -- on the engine's own systems
+ * (from the current state, which is saved and restored, so tuning leaves no
+ * trace), then keeps the fastest.  chosen / ms_per_step (optional) receive the
+ * winning variant and its time per step; lj_engine_pair_kernels tells the
+ * per-system choice. */
 int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step);
 
 /* Mean duration in milliseconds of each step-loop kernel over the last

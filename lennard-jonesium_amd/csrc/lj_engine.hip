@@ -80,7 +80,12 @@ struct lj_engine {
     int ncell_total = 0, ntile = 0;
     int ncblk = 0;                  // cell-space blocks (tiled pair kernel)
     int pair_variant = LJ_PAIR_CELL_TILED;
-    bool last_pair_cells = false;   // partial sums of the last pair launch are cell-space
+    bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
+    int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
+    double split_occupancy = 4.0;   // LJ_PAIR_BY_OCCUPANCY: tiled at >= this many particles per cell
+    int* d_cblk_list = nullptr;     // cell-space blocks of the tiled systems (when split)
+    int* d_blk_list = nullptr;      // particle-space blocks of the other systems (when split)
+    int n_cblk_list = 0, n_blk_list = 0;
 
     SysDev* d_sys = nullptr;
     int* d_blk_sys = nullptr;
@@ -95,7 +100,8 @@ struct lj_engine {
     float4* d_lpos = nullptr;
     int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
     int* d_cell_xyz = nullptr;   // per cell: x | y << 10 | z << 20
-    double* d_partial = nullptr;
+    double* d_partial = nullptr;     // particle-space block sums
+    double* d_partial_c = nullptr;   // cell-space block sums (tiled pair kernels)
     double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
     int* d_step = nullptr;
     int* d_pending = nullptr;   // per system: a step awaits its finalize
@@ -154,6 +160,10 @@ struct lj_engine {
         p.scell = d_scell;
         p.lpos = d_lpos;
         p.partial = d_partial;
+        p.partial_c = d_partial_c;
+        p.skip_tiled = 0;
+        p.cblk_list = nullptr;
+        p.blk_list = nullptr;
         p.pending = d_pending;
         p.cell_xyz = d_cell_xyz;
         p.ncount = d_ncount;
@@ -167,6 +177,7 @@ struct lj_engine {
         ljk::FinalArgs f;
         f.sys = d_sys;
         f.partial = d_partial;
+        f.partial_c = d_partial_c;
         f.time = d_time;
         f.pe = d_pe;
         f.vir = d_vir;
@@ -176,29 +187,25 @@ struct lj_engine {
         f.nsys = nsys;
         f.cap = rec_cap;
         f.mode = mode;
-        f.cell_blocks = last_pair_cells ? 1 : 0;
+        // a step's sums come from the force kernels (per-system layout); a
+        // force-only or measure call's from its own last pair launch
+        f.cell_blocks = (mode == 0 ? sys[0].law != ljk::LAW_NONE : last_pair_force) ? 1 : 0;
         f.pending = d_pending;
-        if (mode == 0) f.cell_blocks = step_uses_cells() ? 1 : 0;
         f.partial2 = d_partial2;
         // first-level parts: about 1024 block sums each, at most kMaxParts
         int nbmax = 1;
-        for (const SysDev& s : sys) nbmax = std::max(nbmax, f.cell_blocks ? s.ncblk : s.nblk);
+        for (const SysDev& s : sys)
+            nbmax = std::max(nbmax, (f.cell_blocks && s.tiled) ? s.ncblk : s.nblk);
         f.nf = std::min(kMaxParts, (nbmax + 1023) / 1024);
         return f;
     }
 
-    // A step's pair kernel is the tiled one (cell-space block sums) unless the
-    // law has no pair force or the per-particle variant is selected.
     // Grids above the single-block scan's reach sum records in two levels.
     bool two_level_records() const
     {
         return has_cells && ncell_total > ljk::kScanOneMax * 4 * ljk::kScanOneThreads;
     }
 
-    bool step_uses_cells() const
-    {
-        return sys[0].law != ljk::LAW_NONE && pair_variant == LJ_PAIR_CELL_TILED;
-    }
 };
 
 namespace {
@@ -326,50 +333,81 @@ int enqueue_cells(lj_engine* e, bool integrate)
 template <int PREC, int LAW, bool KICK>
 void launch_pair(lj_engine* e, const ljk::StepArgs& p)
 {
-    launch(e, ljk::k_pair<PREC, LAW, KICK>, dim3(e->nblk), dim3(kBlock), p);
+    const int grid = p.blk_list ? e->n_blk_list : e->nblk;
+    launch(e, ljk::k_pair<PREC, LAW, KICK>, dim3(grid), dim3(kBlock), p);
 }
 
 template <int PREC, int LAW, bool KICK>
 void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
+    const int grid = p.cblk_list ? e->n_cblk_list : e->ncblk;
     if constexpr (PREC == ljk::MIXED)
-        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
+        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(grid), dim3(kBlock), p);
     else
-        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(e->ncblk), dim3(kBlock), p);
+        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(kBlock), p);
 }
 
 template <int LAW, bool KICK>
-void dispatch_pair(lj_engine* e, const ljk::StepArgs& p, bool mixed, bool cells)
+void launch_tiled(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 {
-    if (cells) {
-        if (mixed) launch_pair_cells<ljk::MIXED, LAW, KICK>(e, p);
-        else launch_pair_cells<ljk::FP64, LAW, KICK>(e, p);
-    } else {
-        if (mixed) launch_pair<ljk::MIXED, LAW, KICK>(e, p);
-        else launch_pair<ljk::FP64, LAW, KICK>(e, p);
-    }
+    if (mixed) launch_pair_cells<ljk::MIXED, LAW, KICK>(e, p);
+    else launch_pair_cells<ljk::FP64, LAW, KICK>(e, p);
 }
 
-// Pair forces (or, for LJ_LAW_NONE engines / measure, just the sums).
+template <int LAW, bool KICK>
+void launch_particle(lj_engine* e, const ljk::StepArgs& p, bool mixed)
+{
+    if (mixed) launch_pair<ljk::MIXED, LAW, KICK>(e, p);
+    else launch_pair<ljk::FP64, LAW, KICK>(e, p);
+}
+
+// Pair forces (or, for LJ_LAW_NONE engines / measure, just the sums): the
+// tiled kernel for systems flagged tiled, the per-particle kernel for the rest.
 int enqueue_pair(lj_engine* e, bool kick, bool measure_only)
 {
-    const ljk::StepArgs p = e->args(e->cur);
+    ljk::StepArgs p = e->args(e->cur);
     const int law = measure_only ? ljk::LAW_NONE : e->sys[0].law;
     const bool mixed = e->precision == LJ_PRECISION_MIXED;
-    const bool cells = law != ljk::LAW_NONE && e->pair_variant == LJ_PAIR_CELL_TILED;
-    e->last_pair_cells = cells;
-    return timed(e, 5, "pair", [&] {
-        if (law == ljk::LAW_NONE) {
+    e->last_pair_force = law != ljk::LAW_NONE;
+    if (law == ljk::LAW_NONE)
+        return timed(e, 5, "pair", [&] {
             if (kick) launch_pair<ljk::FP64, ljk::LAW_NONE, true>(e, p);
             else launch_pair<ljk::FP64, ljk::LAW_NONE, false>(e, p);
-        } else if (law == ljk::LAW_LJ) {
-            if (kick) dispatch_pair<ljk::LAW_LJ, true>(e, p, mixed, cells);
-            else dispatch_pair<ljk::LAW_LJ, false>(e, p, mixed, cells);
+        });
+    auto tiled = [&] {
+        if (law == ljk::LAW_LJ) {
+            if (kick) launch_tiled<ljk::LAW_LJ, true>(e, p, mixed);
+            else launch_tiled<ljk::LAW_LJ, false>(e, p, mixed);
         } else {
-            if (kick) dispatch_pair<ljk::LAW_CONSTANT, true>(e, p, mixed, cells);
-            else dispatch_pair<ljk::LAW_CONSTANT, false>(e, p, mixed, cells);
+            if (kick) launch_tiled<ljk::LAW_CONSTANT, true>(e, p, mixed);
+            else launch_tiled<ljk::LAW_CONSTANT, false>(e, p, mixed);
         }
-    });
+    };
+    auto particle = [&] {
+        if (law == ljk::LAW_LJ) {
+            if (kick) launch_particle<ljk::LAW_LJ, true>(e, p, mixed);
+            else launch_particle<ljk::LAW_LJ, false>(e, p, mixed);
+        } else {
+            if (kick) launch_particle<ljk::LAW_CONSTANT, true>(e, p, mixed);
+            else launch_particle<ljk::LAW_CONSTANT, false>(e, p, mixed);
+        }
+    };
+    // with both kernels in use, each launches only over its own systems' blocks
+    const bool split = e->n_tiled > 0 && e->n_tiled < e->nsys;
+    int rc = LJ_OK;
+    if (e->n_tiled > 0) {
+        if (split) p.cblk_list = e->d_cblk_list;
+        rc = timed(e, 5, "pair", tiled);
+        p.cblk_list = nullptr;
+    }
+    if (!rc && e->n_tiled < e->nsys) {
+        if (split) {
+            p.skip_tiled = 1;
+            p.blk_list = e->d_blk_list;
+        }
+        rc = split ? timed(e, 7, "pair_particle", particle) : timed(e, 5, "pair", particle);
+    }
+    return rc;
 }
 
 int enqueue_finalize(lj_engine* e, int mode)
@@ -470,6 +508,44 @@ int build_graph(lj_engine* e)
     return LJ_OK;
 }
 
+// Per-system pair kernel from the engine's policy: all tiled, all per-particle,
+// or tiled where the mean cell occupancy N / cells reaches split_occupancy.
+// Uploads the flags; the step graphs are rebuilt on their next use.
+int assign_pair_kernels(lj_engine* e)
+{
+    int nt = 0;
+    bool changed = false;
+    for (SysDev& s : e->sys) {
+        int t = e->pair_variant == LJ_PAIR_CELL_TILED ? 1 : 0;
+        if (e->pair_variant == LJ_PAIR_BY_OCCUPANCY)
+            t = (s.ncell > 0 && (double)s.n >= e->split_occupancy * (double)s.ncell) ? 1 : 0;
+        if (!e->has_cells || s.law == ljk::LAW_NONE) t = 0;
+        changed |= t != s.tiled;
+        s.tiled = t;
+        nt += t;
+    }
+    e->n_tiled = nt;
+    if (changed) {
+        drop_graphs(e);
+        std::vector<int> cl, bl;
+        for (const SysDev& s : e->sys) {
+            if (s.tiled)
+                for (int b = 0; b < s.ncblk; ++b) cl.push_back(s.cblk_first + b);
+            else
+                for (int b = 0; b < s.nblk; ++b) bl.push_back(s.blk_first + b);
+        }
+        e->n_cblk_list = (int)cl.size();
+        e->n_blk_list = (int)bl.size();
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        HIP_TRY(hipMemcpy(e->d_sys, e->sys.data(), e->nsys * sizeof(SysDev), hipMemcpyHostToDevice));
+        if (!cl.empty())
+            HIP_TRY(hipMemcpy(e->d_cblk_list, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!bl.empty())
+            HIP_TRY(hipMemcpy(e->d_blk_list, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    return LJ_OK;
+}
+
 int check_engine(const lj_engine* e) { return e ? LJ_OK : fail(LJ_E_INVALID, "null engine"); }
 
 int check_sys(const lj_engine* e, int sys)
@@ -542,6 +618,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         }
         s.n = d.particle_count;
         s.active = 1;
+        s.tiled = (d.force_law != LJ_LAW_NONE && e->has_cells) ? 1 : 0;   // default: all tiled
         s.law = d.force_law;
         s.periodic = d.periodic ? 1 : 0;
         s.dt = d.time_delta;
@@ -614,6 +691,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         e->sys.push_back(s);
     }
     e->nslot = atom_off;
+    for (const SysDev& s : e->sys) e->n_tiled += s.tiled;
     e->nblk = blk;
     e->ncblk = cblk;
     e->ncell_total = cell_off;
@@ -637,7 +715,9 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     const size_t ns = e->nslot;
     if ((rc = dev_alloc(&e->d_sys, nsys)) || (rc = dev_alloc(&e->d_blk_sys, blk)) ||
         (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_force, 3 * ns)) ||
-        (rc = dev_alloc(&e->d_partial, 4 * (size_t)std::max(blk, cblk))) ||
+        (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
+        (rc = dev_alloc(&e->d_partial_c, 4 * (size_t)cblk)) ||
+        (rc = dev_alloc(&e->d_cblk_list, (size_t)cblk)) || (rc = dev_alloc(&e->d_blk_list, (size_t)blk)) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
         (rc = dev_alloc(&e->d_pending, nsys)) ||
@@ -711,7 +791,7 @@ int lj_engine_destroy(lj_engine* e)
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
                     e->d_perm_id,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
-                    e->d_partial, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
+                    e->d_partial, e->d_partial_c, e->d_cblk_list, e->d_blk_list, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
                     e->d_measure, e->d_rdf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -1074,12 +1154,30 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
 {
     int rc = check_engine(e);
     if (rc) return rc;
-    if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED)
+    if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED &&
+        variant != LJ_PAIR_BY_OCCUPANCY)
         return fail(LJ_E_INVALID, "unknown pair kernel variant");
-    if (variant != e->pair_variant && (e->graph2 || e->graphN)) {
-        drop_graphs(e);
-    }
     e->pair_variant = variant;
+    return assign_pair_kernels(e);
+}
+
+int lj_engine_set_pair_split(lj_engine* e, double min_occupancy)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!(min_occupancy > 0)) return fail(LJ_E_INVALID, "min_occupancy must be > 0");
+    e->split_occupancy = min_occupancy;
+    e->pair_variant = LJ_PAIR_BY_OCCUPANCY;
+    return assign_pair_kernels(e);
+}
+
+int lj_engine_pair_kernels(lj_engine* e, int* per_system)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!per_system) return fail(LJ_E_INVALID, "null output");
+    for (int k = 0; k < e->nsys; ++k)
+        per_system[k] = e->sys[k].tiled ? LJ_PAIR_CELL_TILED : LJ_PAIR_PER_PARTICLE;
     return LJ_OK;
 }
 
@@ -1131,11 +1229,30 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     };
     hipError_t he = save();
     if (he != hipSuccess) { (void)hipFree(sid); return fail(LJ_E_HIP, "autotune: save failed"); }
-    const int variants[2] = {LJ_PAIR_CELL_TILED, LJ_PAIR_PER_PARTICLE};
+    // candidates: all tiled, all per-particle, and splits by cell occupancy
+    // whose tiled sets differ from those and from each other
+    struct Candidate {
+        int variant;
+        double occupancy;
+    };
+    std::vector<Candidate> cands = {{LJ_PAIR_CELL_TILED, 0.0}, {LJ_PAIR_PER_PARTICLE, 0.0}};
+    {
+        std::vector<int> seen = {0, e->nsys};
+        for (double thr : {1.5, 3.0, 6.0, 12.0}) {
+            int nt = 0;
+            for (const SysDev& s : e->sys)
+                nt += (s.ncell > 0 && (double)s.n >= thr * (double)s.ncell) ? 1 : 0;
+            if (std::find(seen.begin(), seen.end(), nt) != seen.end()) continue;
+            seen.push_back(nt);
+            cands.push_back({LJ_PAIR_BY_OCCUPANCY, thr});
+        }
+    }
+    const double occ0 = e->split_occupancy;
     double best = 1e300;
-    int best_v = e->pair_variant;
-    for (int v : variants) {
-        if ((rc = lj_engine_set_pair_kernel(e, v))) break;
+    Candidate best_c = {e->pair_variant, occ0};
+    for (const Candidate& c : cands) {
+        e->split_occupancy = c.variant == LJ_PAIR_BY_OCCUPANCY ? c.occupancy : occ0;
+        if ((rc = lj_engine_set_pair_kernel(e, c.variant))) break;
         if ((rc = lj_engine_step(e, 2, nullptr))) break;   // warm-up (and graph build)
         HIP_TRY(hipEventRecord(e->ev0, e->stream));
         if ((rc = lj_engine_step(e, steps, nullptr))) break;
@@ -1143,15 +1260,16 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         HIP_TRY(hipEventSynchronize(e->ev1));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-        if (ms < best) { best = ms; best_v = v; }
+        if (ms < best) { best = ms; best_c = c; }
         if ((he = restore()) != hipSuccess) break;
     }
     (void)hipFree(sid);
     if (he != hipSuccess) return fail(LJ_E_HIP, "autotune: restore failed");
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if ((rc = lj_engine_set_pair_kernel(e, best_v))) return rc;
-    if (chosen) *chosen = best_v;
+    e->split_occupancy = best_c.variant == LJ_PAIR_BY_OCCUPANCY ? best_c.occupancy : occ0;
+    if ((rc = lj_engine_set_pair_kernel(e, best_c.variant))) return rc;
+    if (chosen) *chosen = best_c.variant;
     if (ms_per_step) *ms_per_step = best / steps;
     return LJ_OK;
 }

@@ -50,7 +50,7 @@ struct SysDev {
     int cblk_first;    // first cell-space block (kCellsPerBlock cells each)
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
-    int pad_;
+    int tiled;         // pair forces from the tiled kernel (1) or the per-particle kernel (0)
     double L[3];       // box
     double q[3];       // nc / L (the reference forms this quotient first)
     double a[3];       // L / nc (cell side, used for cell-local coordinates)
@@ -197,7 +197,11 @@ struct StepArgs {
     int* perm_id;                       // particle id of each perm entry
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
-    double* partial;                    // per block: sum v^2, PE, W, sum d^2
+    double* partial;                    // per particle-space block: sum v^2, PE, W, sum d^2
+    double* partial_c;                  // the same per cell-space block (tiled pair kernels)
+    int skip_tiled;                     // per-particle pair kernel: leave tiled systems alone
+    const int* cblk_list;               // tiled pair kernels: cell-space blocks to run (null: all)
+    const int* blk_list;                // per-particle pair kernel: blocks to run (null: all)
     int* pending;                       // per system: a step's block sums await the finalize
     const int* cell_xyz;                // per cell: x | y << 10 | z << 20 (cell coordinates)
     int* ncount;                        // optional: in-cutoff neighbours per slot
@@ -211,7 +215,8 @@ struct StepArgs {
 // ---------------------------------------------------------------------------
 struct FinalArgs {
     const SysDev* sys;
-    const double* partial;
+    const double* partial;    // particle-space block sums
+    const double* partial_c;  // cell-space block sums (systems with tiled pair forces)
     double* time;      // per system
     double* pe;        // per system (last force calculation)
     double* vir;
@@ -221,7 +226,7 @@ struct FinalArgs {
     int nsys;
     int cap;
     int mode;
-    int cell_blocks;   // partial sums come from cell-space blocks (tiled pair kernel)
+    int cell_blocks;   // 1: a system with tiled pair forces has cell-space sums; 0: all particle-space
     int* pending;      // per system: set by a step's pair kernel, cleared here
     double* partial2;  // [nsys][nf] first-level sums of large grids (see k_scan_tiles)
     int nf;            // first-level parts per system
@@ -656,7 +661,8 @@ __device__ __forceinline__ void pair_fp64(const SysDev& S, double dx, double dy,
 // b % 8), and each XCD has its own L2.  Renumbering so that XCD x walks one
 // contiguous range of logical blocks keeps each XCD on one slab of
 // neighbouring cells, so a cell's neighbours are fetched into one L2 instead
-// of all eight.  Bijective on [0, gridDim.x).
+// of all eight.  Bijective on [0, gridDim.x).  (A pair launch over a subset
+// of the systems maps this index through its ordered block list.)
 constexpr int kXcds = 8;
 
 __device__ __forceinline__ int xcd_slab_block()
@@ -669,8 +675,9 @@ __device__ __forceinline__ int xcd_slab_block()
 template <int PREC, int LAW, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 {
-    const int b = xcd_slab_block();
+    const int b = p.blk_list ? p.blk_list[xcd_slab_block()] : xcd_slab_block();
     const SysDev& S = p.sys[p.blk_sys[b]];
+    if (p.skip_tiled && S.tiled) return;   // this system's forces come from a tiled kernel
     const int s = b * kBlock + threadIdx.x;
     const bool active = (s - S.atom_off) < S.n && (!KICK || S.active);
     double fx = 0, fy = 0, fz = 0, pe = 0, w = 0;
@@ -889,9 +896,10 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     __shared__ float4 tb_sh[NW][27];    // image shift + chunk origin
     __shared__ int selfe[NW][64];       // staged entry of each chunk particle (-1: none)
 
-    const int b = xcd_slab_block();
+    const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
+    if (!S.tiled) return;   // this system's forces come from the per-particle kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
@@ -1165,7 +1173,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     block_sum4_lds(wacc, reinterpret_cast<double*>(st[wid]), out);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+        for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
         if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
@@ -1219,9 +1227,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
     __shared__ double tb_shd[NW][27][3];       // image shift i L of each neighbour cell
     __shared__ double tb_rel[NW][27][3];       // i L - o (o: chunk centre)
 
-    const int b = xcd_slab_block();
+    const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
+    if (!S.tiled) return;   // this system's forces come from the per-particle kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
@@ -1457,7 +1466,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
     block_sum4_lds(wacc, sdw, out);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p.partial[4 * b + q] = out[q];
+        for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
         if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
@@ -1523,28 +1532,34 @@ __device__ void finalize_range(const FinalArgs& f, int k, const double* part, in
 }
 
 // Reduces system k's block sums (pair-kernel partials) into its record.
+// System k's block sums: cell-space for tiled pair forces, else particle-space.
+__device__ __forceinline__ const double* block_sums(const FinalArgs& f, const SysDev& S, int& nb)
+{
+    const bool cells = f.cell_blocks && S.tiled;
+    nb = cells ? S.ncblk : S.nblk;
+    return cells ? f.partial_c + 4 * (size_t)S.cblk_first : f.partial + 4 * (size_t)S.blk_first;
+}
+
 template <int NT>
 __device__ void finalize_system(const FinalArgs& f, int k)
 {
-    const SysDev& S = f.sys[k];
-    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
-    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
-    finalize_range<NT>(f, k, f.partial + 4 * (size_t)b0, nb);
+    int nb;
+    const double* part = block_sums(f, f.sys[k], nb);
+    finalize_range<NT>(f, k, part, nb);
 }
 
 // First level for large grids: part r of f.nf contiguous parts of system k's
 // block sums, summed by one kBlock-thread block into f.partial2.
 __device__ void finalize_part(const FinalArgs& f, int k, int r)
 {
-    const SysDev& S = f.sys[k];
-    const int b0 = f.cell_blocks ? S.cblk_first : S.blk_first;
-    const int nb = f.cell_blocks ? S.ncblk : S.nblk;
+    int nb;
+    const double* part = block_sums(f, f.sys[k], nb);
     const int chunk = (nb + f.nf - 1) / f.nf;
     const int lo = min(nb, r * chunk), hi = min(nb, lo + chunk);
     double acc[4] = {0, 0, 0, 0};
 #pragma unroll 8
     for (int b = lo + (int)threadIdx.x; b < hi; b += kBlock) {
-        const double4 v = reinterpret_cast<const double4*>(f.partial)[b0 + b];
+        const double4 v = reinterpret_cast<const double4*>(part)[b];
         acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
     }
     double out[4];

@@ -84,6 +84,8 @@ def engine():
         "lj_engine_adjoint": (c_i, [V, c_i, c_i, c_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
+        "lj_engine_set_pair_split": (c_i, [V, c_d]),
+        "lj_engine_pair_kernels": (c_i, [V, P_i]),
         "lj_engine_autotune": (c_i, [V, c_i, P_i, P_d]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),
         "lj_engine_kernel_times": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_char_p), P_d, P_i]),

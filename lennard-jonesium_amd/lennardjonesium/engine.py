@@ -19,7 +19,7 @@ from lennardjonesium._native import check
 
 LAWS = {"lj": 0, "constant": 1, "none": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
-PAIR_KERNELS = {"particle": 0, "tiled": 1}
+PAIR_KERNELS = {"particle": 0, "tiled": 1, "split": 2}
 THERMO_FIELDS = ("time", "kinetic_energy", "potential_energy", "total_energy", "virial",
                  "temperature", "mean_square_displacement")
 
@@ -218,8 +218,20 @@ class Engine:
         check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
 
     def set_pair_kernel(self, variant):
-        """'tiled' (one wave per home cell, LDS-staged neighbours) or 'particle'."""
+        """'tiled' (one wave per home cell, LDS-staged neighbours), 'particle'
+        (one thread per particle) or 'split' (per system, by cell occupancy)."""
         check(self.lib.lj_engine_set_pair_kernel(self.h, PAIR_KERNELS[variant]))
+
+    def set_pair_split(self, min_occupancy):
+        """Per-system kernels: tiled where N / cells >= min_occupancy."""
+        check(self.lib.lj_engine_set_pair_split(self.h, float(min_occupancy)))
+
+    def pair_kernels(self):
+        """The kernel each system uses now ('tiled' or 'particle')."""
+        out = np.zeros(self.nsys, dtype=np.int32)
+        check(self.lib.lj_engine_pair_kernels(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        names = {k: n for n, k in PAIR_KERNELS.items()}
+        return [names[int(v)] for v in out]
 
     def autotune(self, steps=20):
         """Times each pair-kernel variant on this workload and keeps the fastest;

@@ -259,8 +259,12 @@ def test_trajectory_tracks_oracle_and_graphs_are_exact():
     assert rel[:100].max() < 1e-11 and rel.max() < 1e-6
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "split"])
 def test_batched_replicas_match_single_runs(kernel):
+    """One engine for three uneven replicas gives each replica's single-engine
+    records bit for bit; 'split' runs the tiled kernel for two replicas and
+    the per-particle kernel for the dilute one (threshold 10 per cell) in the
+    same step."""
     eng = _engine_mod()
     cases = [(500, 0.8, 0.7, 3), (256, 0.3, 1.5, 4), (864, 1.1, 0.9, 5)]
     specs, states = [], []
@@ -268,18 +272,52 @@ def test_batched_replicas_match_single_runs(kernel):
         st, L = ljref.initial_condition(n, rho, T, seed=seed)
         specs.append(eng.SystemSpec(n, L))
         states.append((st, L))
-    e = eng.Engine(specs, pair_kernel=kernel)
+
+    def engine(sp):
+        e = eng.Engine(sp, pair_kernel="tiled" if kernel == "split" else kernel)
+        if kernel == "split":
+            e.set_pair_split(10.0)
+        return e
+
+    e = engine(specs)
+    if kernel == "split":
+        assert e.pair_kernels() == ["tiled", "particle", "tiled"]
     for k, (st, _) in enumerate(states):
         e.set_state(k, st.pos, st.vel)
     rec = e.step(100)
     for k, (st, L) in enumerate(states):
-        single = eng.Engine([specs[k]], pair_kernel=kernel)
+        single = engine([specs[k]])
         single.set_state(0, st.pos, st.vel)
         r1 = single.step(100)[:, 0, :]
         assert np.array_equal(rec[:, k, :], r1)      # batching does not change the arithmetic
         ref = ljref.run(st.copy(), L, 0.005, 100)
         assert (np.abs(r1 - ref) / np.abs(ref)).max() < 1e-10
         single.close()
+    e.close()
+
+
+def test_split_kernels_neighbour_counts_exact():
+    """FP64 in-cutoff counts per particle are the oracle's with both kernels
+    active in one engine (a dense and a dilute replica)."""
+    eng = _engine_mod()
+    cases = [(4000, 0.8, 1.0, 7), (2048, 0.1, 1.5, 8)]
+    specs, states = [], []
+    for n, rho, T, seed in cases:
+        st, L = ljref.initial_condition(n, rho, T, seed=seed)
+        st.pos = ljref.wrap(st.pos + np.random.default_rng(seed).normal(0, 0.05, st.pos.shape), L)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((st, L))
+    e = eng.Engine(specs)
+    e.set_pair_split(4.0)
+    assert e.pair_kernels() == ["tiled", "particle"]
+    for k, (st, _) in enumerate(states):
+        e.set_state(k, st.pos, st.vel)
+    for k, (st, L) in enumerate(states):
+        pos = st.pos
+        assert np.array_equal(e.neighbor_counts(k), _oracle_counts(pos, L))
+        F = e.get_state(k)["force"]
+        Fr, _, _ = ljref.forces(pos, L)
+        assert force_err(F, Fr) < FTOL
     e.close()
 
 
