@@ -201,7 +201,7 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
  * cell with the neighbour cells staged in LDS; LJ_PAIR_PER_PARTICLE runs one
  * thread per particle over the 27 neighbour cells; LJ_PAIR_BY_OCCUPANCY picks
  * per system: tiled where the mean cell occupancy N / cells is at least the
- * threshold of lj_engine_set_pair_split (default 4), per-particle elsewhere
+ * threshold of lj_engine_set_pair_split (default 6), per-particle elsewhere
  * (dilute replicas of a sweep), both kernels running in the same step.  All
  * give the same pair set and, in FP64, the same per-pair arithmetic.  No
  * counterpart in the reference (its single pair filter,

@@ -420,6 +420,13 @@ int run_batch(lj_simulation** sims, int nsim, int echo, int* status)
     lj_engine* e = nullptr;
     int rc = lj_engine_create(desc.data(), nsim, prec, &e);
     if (rc) return fail(rc, lj_last_error());
+    // tiled pair kernel for dense systems, per-particle for dilute ones: a
+    // rule of each system alone, so a batch and a single run of the same
+    // system give identical output
+    if ((rc = lj_engine_set_pair_kernel(e, LJ_PAIR_BY_OCCUPANCY))) {
+        lj_engine_destroy(e);
+        return fail(rc, lj_last_error());
+    }
     std::vector<Run> runs(nsim);
     for (int k = 0; k < nsim; ++k) {
         runs[k].sim = sims[k];

@@ -82,7 +82,7 @@ struct lj_engine {
     int pair_variant = LJ_PAIR_CELL_TILED;
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
-    double split_occupancy = 4.0;   // LJ_PAIR_BY_OCCUPANCY: tiled at >= this many particles per cell
+    double split_occupancy = 6.0;   // LJ_PAIR_BY_OCCUPANCY: tiled at >= this many particles per cell
     int* d_cblk_list = nullptr;     // cell-space blocks of the tiled systems (when split)
     int* d_blk_list = nullptr;      // particle-space blocks of the other systems (when split)
     int n_cblk_list = 0, n_blk_list = 0;
