@@ -81,6 +81,17 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// 1 / x for 0 < x < ~1e300 from the hardware reciprocal refined by two
+// Newton steps (error within an ulp or two; no IEEE special cases needed).
+__device__ __forceinline__ double rcp_f64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
 // One DPP lane move of a double (two 32-bit halves).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
@@ -1245,6 +1256,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
         const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
         float4* sw = st[wid];
         const double rc2d = S.rc2;
+        const double sig2 = S.sigma2;
+        const double cwd = -2.0 * S.beta * S.inv_rc2s;
         const float rc2f = S.rc2f;
         const float keepf = rc2f + 2e-3f;   // bounding-box pruning, wider than the phase-1 margin
 
@@ -1340,6 +1353,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
             fetch(lane, nq, nx, ny, nz);
             int g0 = 0;
             double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, w = 0.0;
+            double s12a = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of s^12, s^6, r^2
             int nct = 0;
             while (g0 < total) {
                 // ---- stage the surviving neighbours of the next candidates, 64 at a time
@@ -1422,11 +1436,35 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                     const double dz = (zi - sdw[3 * e + 2]) - tb_shd[wid][q][2];
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (r2 < rc2d && r2 > 0.0) {
-                        pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
                         ++nct;
+                        if (LAW == LAW_LJ) {
+                            // s = (sigma / r)^6; W = 48 s^2 - 24 s - 2 beta u, F = W sep / r^2
+                            const double inv = rcp_f64(r2);
+                            const double t = inv * sig2;
+                            const double s6 = t * t * t;
+                            const double s12 = s6 * s6;
+                            s12a += s12;
+                            s6a += s6;
+                            r2a += r2;
+                            const double ww = __builtin_fma(48.0, s12, __builtin_fma(-24.0, s6, cwd * r2));
+                            const double fm = ww * inv;
+                            fx = __builtin_fma(fm, dx, fx);
+                            fy = __builtin_fma(fm, dy, fy);
+                            fz = __builtin_fma(fm, dz, fz);
+                        } else {
+                            pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+                        }
                     }
                 }
                 wave_lds_sync();
+            }
+            if (LAW == LAW_LJ) {
+                // V = 4 s^2 - 4 s + (alpha - beta) + beta u, W = 48 s^2 - 24 s - 2 beta u,
+                // times epsilon (physics/lennard_jones_force.cpp:107-121)
+                const double u_sum = r2a * S.inv_rc2s;
+                pe = S.eps * (4.0 * s12a - 4.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
+                w = S.eps * (48.0 * s12a - 24.0 * s6a - 2.0 * S.beta * u_sum);
+                fx *= S.eps; fy *= S.eps; fz *= S.eps;
             }
             // ---- sum the slices of each particle in a fixed order
             sdw[4 * lane] = fx;
