@@ -68,19 +68,24 @@ def kernel_times(e, steps=40):
     return kt
 
 
-def read_traffic(name):
-    """HBM bytes per launch of the pair kernel from the committed rocprofv3
-    --pmc summary (a line 'traffic_bytes_per_launch: <bytes>'), or None."""
+def read_profile_value(name, key):
+    """A 'key: <number>' line of a committed rocprofv3 summary, or None."""
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1])
             except ValueError:
                 return None
     return None
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the pair kernel from the committed rocprofv3
+    --pmc summary (a line 'traffic_bytes_per_launch: <bytes>'), or None."""
+    return read_profile_value(name, "traffic_bytes_per_launch")
 
 
 def cpu_baseline(n, rho, T, seed, steps, rc=2.5, dt=0.005):
@@ -149,6 +154,11 @@ def liquid(args, eng):
     flops = pair_flops(c_half, p_half)
     achieved = flops / (ktm["pair"] * 1e-3) / 1e12
     traffic = read_traffic("pair_mixed_256k_traffic.txt")
+    # VALU issue: wave-instructions per launch (committed SQ counters) over the
+    # CUs' issue capacity in the measured duration (one wave-instruction per CU
+    # per clock at 2.4 GHz, 256 CUs)
+    valu = read_profile_value("step_kernels_256k_sq_counters.md", "valu_insts_per_launch")
+    issue = valu / (ktm["pair"] * 1e-3 * 256 * 2.4e9) if valu else None
     cpu = None
     if not args.no_cpu:
         rate_cpu, dt_cpu = cpu_baseline(n, rho, T, seed, args.cpu_steps)
@@ -176,7 +186,8 @@ def liquid(args, eng):
                      "kernel": ("k_pair_cells_mixed<LJ,KICK>" if args.pair_kernel == "tiled"
                                 else "k_pair<MIXED,LJ,KICK>"),
                      "kernel_ms": ktm["pair"], "flops_per_launch": flops,
-                     "c_half": c_half, "p_half": p_half},
+                     "c_half": c_half, "p_half": p_half,
+                     "valu_insts_per_launch": valu, "valu_issue_frac": issue},
         "kernel_ms": ktm,
         "cpu_baseline": cpu,
     }
