@@ -651,6 +651,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         s.rc2f = (float)s.rc2;
         s.inv_s2f = (float)(1.0 / s.sigma2);
         s.cwf = (float)(-2.0 * s.beta * s.inv_rc2s);
+        s.cw24f = (float)(-2.0 * s.beta * s.inv_rc2s / 24.0);
         if (d.force_law != LJ_LAW_NONE) {
             for (int a = 0; a < 3; ++a) {
                 if (!(d.box[a] >= rc_abs) || !(rc_abs > 0)) {

@@ -65,7 +65,7 @@ struct SysDev {
     float rc2f;        // (float) rc2
     float inv_s2f;     // (float) (1 / sigma2)
     float cwf;         // (float) (-2 beta inv_rc2s)
-    float pad_f;
+    float cw24f;       // (float) (-2 beta inv_rc2s / 24)
     // image shift of a neighbour cell, per axis: shf[a][3 (o + 1) + (i + 1)] =
     // (float) ((i nc - o) a - i L) for step o and lattice image i in {-1, 0, 1}
     float shf[3][9];
@@ -566,32 +566,48 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     const int u = b * kBlock + threadIdx.x;
     if (u - S.atom_off >= S.n) return;
     const int old = p.perm[u];
-    const int c = p.cell_of[old];
     const int me = p.perm_id[u];
-    const int c0 = p.start[c], c1 = p.start[c + 1];
-    if (u == c0) p.count[c] = 0;   // ready for the next counting pass (tail cells keep theirs)
-    int rank = 0;
-    for (int q = c0; q < c1; ++q) rank += (p.perm_id[q] < me) ? 1 : 0;
-    const int dst = c0 + rank;
+    const int c = p.cell_of[old];
+    // The particle's state is loaded before the rank search so that its
+    // latency overlaps the search (every slot has a force, zero before the
+    // first calculation).
     double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
     double vx = p.vel.x[old], vy = p.vel.y[old], vz = p.vel.z[old];
     double dx = p.disp.x[old], dy = p.disp.y[old], dz = p.disp.z[old];
+    double fx = 0., fy = 0., fz = 0.;
+    if (INTEGRATE) { fx = p.force.x[old]; fy = p.force.y[old]; fz = p.force.z[old]; }
+    const int c0 = p.start[c], c1 = p.start[c + 1];
+    int cx = 0, cy = 0, cz = 0;
+    if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+    // rank by id inside the cell: eight loads in flight per round (indices
+    // past the cell's end re-read its last entry and are not counted)
+    int rank = 0;
+    for (int q = c0; q < c1; q += 8) {
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p.perm_id[min(q + k, c1 - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rank += (q + k < c1 && v[k] < me) ? 1 : 0;
+    }
+    const int dst = c0 + rank;
     if (INTEGRATE && S.active) {
         double ix, iy, iz;
-        advance(S, x, y, z, vx, vy, vz, p.force.x[old], p.force.y[old], p.force.z[old], ix, iy, iz);
+        advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
         dx = dx + ix; dy = dy + iy; dz = dz + iz;
     }
+    float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (MIXED_)
+        lp = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                         (float)(z - cz * S.a[2]), 0.0f);
     p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
     p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
     p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
     p.id_n[dst] = me;
     p.scell[dst] = c;
-    if (MIXED_) {
-        int cx, cy, cz;
-        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
-        p.lpos[dst] = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
-                                  (float)(z - cz * S.a[2]), 0.0f);
-    }
+    if (MIXED_) p.lpos[dst] = lp;
+    // ready for the next counting pass (tail cells keep theirs); stored last,
+    // after every read of the system's parameters
+    if (u == c0) p.count[c] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -860,9 +876,10 @@ __device__ __forceinline__ void block_sum4_lds(const double v[4], double* sc, do
 //     and shifts the outcome into a 64-bit mask; the entry of the particle
 //     itself is recorded while staging and its bit is cleared, so phase 2
 //     runs the pair law on every set bit without a test;
-//   * phase 2 accumulates the force and, per stage, sum s^12, sum s^6 and
-//     sum r^2, from which V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u and
-//     W = 48 s^12 - 24 s^6 - 2 beta u (u = r^2 / rc^2) follow in double.
+//   * phase 2 accumulates the force and, per stage, sum g, sum s^6 and
+//     sum r^2 with g = 48 s^12 - 24 s^6, from which V = 4 s^12 - 4 s^6 +
+//     (alpha - beta) + beta u = g / 12 - 2 s^6 + (alpha - beta) + beta u and
+//     W = g - 2 beta u (u = r^2 / rc^2) follow in double.
 // ---------------------------------------------------------------------------
 constexpr int kStageM = 256;   // staged candidates per stage
 constexpr int kRowsM = 324;    // >= max over slice counts S of S * rows(S) + 3
@@ -901,11 +918,17 @@ template <int LAW, bool KICK>
 __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
 {
     constexpr int NW = kBlock / 64;
-    __shared__ float4 st[NW][kRowsM];
-    __shared__ int tb_delta[NW][27];    // slot of a cell's first particle minus its offset
-    __shared__ int tb_off[NW][32];      // entries 27..31 = INT_MAX (search padding)
-    __shared__ float4 tb_sh[NW][27];    // image shift + chunk origin
-    __shared__ int selfe[NW][64];       // staged entry of each chunk particle (-1: none)
+    // One record per wave, so that every per-lane LDS address is one base
+    // register plus an immediate offset.
+    struct alignas(16) WaveLds {
+        float4 st[kRowsM];   // stage rows; slice sums; the block sums' scratch
+        float4 tb_sh[27];    // image shift + chunk origin
+        float4 tb_img[27];   // image shift of each neighbour cell
+        int tb_delta[27];    // slot of a cell's first particle minus its offset
+        int tb_off[32];      // entries 27..31 = INT_MAX (search padding)
+        int selfe[64];       // staged entry of each chunk particle (-1: none)
+    };
+    __shared__ WaveLds wl[NW];
 
     const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
@@ -913,6 +936,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     if (!S.tiled) return;   // this system's forces come from the per-particle kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    WaveLds& W = wl[wid];
     const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
@@ -922,15 +946,14 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
         cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
         const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
-        float4* sw = st[wid];
+        float4* sw = W.st;
         const float rc2f = S.rc2f;
         const float inv_s2 = S.inv_s2f;
-        const float cw = S.cwf;
+        const float cw24 = S.cw24f;
         const float keepf = rc2f + 1e-3f;   // bounding-box pruning, wider than any rounding
 
         // ---- the 27 neighbour cells (reference lattice-image rule), one lane each
         int cntq = 0, j0 = 0;
-        float shx = 0.f, shy = 0.f, shz = 0.f;
         if (lane < 27) {
             const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
             int nbx, ix, nby, iy, nbz, iz;
@@ -940,16 +963,17 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
             cntq = p.start[nb + 1] - j0;
-            shx = S.shf[0][3 * (ox + 1) + ix + 1];
-            shy = S.shf[1][3 * (oy + 1) + iy + 1];
-            shz = S.shf[2][3 * (oz + 1) + iz + 1];
+            // kept in LDS for the chunks of this cell (not in registers)
+            W.tb_img[lane] = make_float4(S.shf[0][3 * (ox + 1) + ix + 1],
+                                            S.shf[1][3 * (oy + 1) + iy + 1],
+                                            S.shf[2][3 * (oz + 1) + iz + 1], 0.f);
         }
         const int incl = wave_incl_scan_dpp(cntq);
         if (lane < 27) {
-            tb_off[wid][lane] = incl - cntq;
-            tb_delta[wid][lane] = j0 - (incl - cntq);
+            W.tb_off[lane] = incl - cntq;
+            W.tb_delta[lane] = j0 - (incl - cntq);
         } else if (lane < 32) {
-            tb_off[wid][lane] = 0x7fffffff;
+            W.tb_off[lane] = 0x7fffffff;
         }
         const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 26, 64));
         // offset of the home cell (offset 13) in the concatenation
@@ -993,8 +1017,10 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             const float4* rowp = sw + sl * SL;
 
             wave_lds_sync();   // the previous chunk's reads of tb_sh / sw are done
-            if (lane < 27)
-                tb_sh[wid][lane] = make_float4(shx + o3[0], shy + o3[1], shz + o3[2], 0.f);
+            if (lane < 27) {
+                const float4 im = W.tb_img[lane];
+                W.tb_sh[lane] = make_float4(im.x + o3[0], im.y + o3[1], im.z + o3[2], 0.f);
+            }
             wave_lds_sync();
 
             // candidate g of the concatenated neighbour cells (clamped to the
@@ -1005,9 +1031,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 int pos = 0;
 #pragma unroll
                 for (int step = 16; step > 0; step >>= 1)
-                    if (tb_off[wid][pos + step] <= gc) pos += step;
+                    if (W.tb_off[pos + step] <= gc) pos += step;
                 q = pos;
-                const unsigned j = (unsigned)(tb_delta[wid][pos] + gc);
+                const unsigned j = (unsigned)(W.tb_delta[pos] + gc);
                 v = *(const float4*)((const char*)p.lpos + j * 16u);
             };
             int nq;
@@ -1016,12 +1042,12 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
             const int self0 = off_home + (i0 - h0);   // concatenation index of the chunk's first particle
             int g0 = 0;
             double pe = 0.0, w = 0.0;              // constant law
-            double s12a = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of s^12, s^6, r^2
+            double ga = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of g, s^6, r^2
             float ax = 0.f, ay = 0.f, az = 0.f;
             int nct = 0;
             while (g0 < total) {
                 // ---- stage the surviving neighbours of the next candidates, 64 at a time
-                if (lane < ni) selfe[wid][lane] = -1;
+                if (lane < ni) W.selfe[lane] = -1;
                 int m = 0;
                 while (g0 < total && m <= cap - 64) {
                     const int cq = nq;
@@ -1029,7 +1055,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                     const int g = g0 + lane;
                     g0 += 64;
                     fetch(g0 + lane, nq, nv);
-                    const float4 sh = tb_sh[wid][cq];
+                    const float4 sh = W.tb_sh[cq];
                     float4 rv;
                     rv.x = cv.x - sh.x;
                     rv.y = cv.y - sh.y;
@@ -1048,7 +1074,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                         sw[e] = rv;
                         // chunk particles occupy [self0, self0 + ni) of the home cell's range
                         const unsigned ic = (unsigned)(g - self0);
-                        if (ic < (unsigned)ni) selfe[wid][ic] = (int)e;
+                        if (ic < (unsigned)ni) W.selfe[ic] = (int)e;
                     }
                     m += __popcll(bal);
                 }
@@ -1064,60 +1090,61 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 if (kmax == 0) continue;
 
                 // ---- phase 1: kmax rows per lane, four per step (rows past kmax are dropped)
+                // Rows are visited from the last group of four down, so that
+                // row r ends up in bit r (the first row is shifted in last).
                 const int k4 = (kmax + 3) & ~3;
                 const int nlo = min(k4, 32), nhi = k4 - nlo;
                 unsigned lo = 0u, hi = 0u;
-                for (int kb = 0; kb < nlo; kb += 4) {
+                for (int kb = 32 + nhi - 4; kb >= 32; kb -= 4) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const float4 d = rowp[kb + u];
-                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
-                        // shift in the sign of v - thr (set iff v < thr)
-                        lo = __builtin_amdgcn_alignbit(lo, (unsigned)__float_as_int(v - thr), 31);
-                    }
-                }
-                for (int kb = 32; kb < 32 + nhi; kb += 4) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 3; u >= 0; --u) {
                         const float4 d = rowp[kb + u];
                         const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
                         // shift in the sign of v - thr (set iff v < thr)
                         hi = __builtin_amdgcn_alignbit(hi, (unsigned)__float_as_int(v - thr), 31);
                     }
                 }
-                // row kb -> bit 63 - kb
-                lo = (nlo == 32) ? lo : (lo << (32 - nlo));
-                hi = (nhi == 0) ? 0u : ((nhi == 32) ? hi : (hi << (32 - nhi)));
-                unsigned long long mask = (((unsigned long long)lo << 32) | hi) & (~0ull << (64 - kmax));
-                const int self_row = selfe[wid][il] - sl * SL;
-                if ((unsigned)self_row < (unsigned)SL) mask &= ~(1ull << (63 - self_row));
+                for (int kb = nlo - 4; kb >= 0; kb -= 4) {
+#pragma unroll
+                    for (int u = 3; u >= 0; --u) {
+                        const float4 d = rowp[kb + u];
+                        const float v = __builtin_fmaf(m2x, d.x, __builtin_fmaf(m2y, d.y, __builtin_fmaf(m2z, d.z, d.w)));
+                        lo = __builtin_amdgcn_alignbit(lo, (unsigned)__float_as_int(v - thr), 31);
+                    }
+                }
+                // row r -> bit r; rows kmax .. k4 - 1 are dropped
+                unsigned long long mask = ((unsigned long long)hi << 32) | lo;
+                if (kmax < 64) mask &= (1ull << kmax) - 1ull;
+                const int self_row = W.selfe[il] - sl * SL;
+                if ((unsigned)self_row < (unsigned)SL) mask &= ~(1ull << self_row);
                 if (!act) mask = 0ull;
                 nct += __popcll(mask);
 
                 // ---- phase 2: pair law on every set bit
-                const float4* top = rowp + 63;
                 if (LAW == LAW_LJ) {
-                    float s12s = 0.f, s6s = 0.f, r2s = 0.f;
+                    // g / 24 = 2 s^12 - s^6 = s^6 (2 s^6 - 1) (inline constants only);
+                    // the force is accumulated as (F/r) / 24 = (g / 24) / r^2 + cw / 24
+                    // and scaled by 24 in double once per particle
+                    float gs = 0.f, s6s = 0.f, r2s = 0.f;
                     while (mask) {
                         const int bit = __builtin_ctzll(mask);
                         mask &= mask - 1;
-                        const float4 d = top[-bit];
+                        const float4 d = rowp[bit];
                         const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         const float inv = __builtin_amdgcn_rcpf(r2);
                         const float t = inv * inv_s2;
                         const float s6 = t * t * t;
-                        const float s12 = s6 * s6;
-                        s12s += s12;
+                        const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
+                        gs += g;
                         s6s += s6;
                         r2s += r2;
-                        const float ww = __builtin_fmaf(48.0f, s12, __builtin_fmaf(-24.0f, s6, r2 * cw));
-                        const float fm = ww * inv;
+                        const float fm = __builtin_fmaf(g, inv, cw24);
                         ax = __builtin_fmaf(fm, dx, ax);
                         ay = __builtin_fmaf(fm, dy, ay);
                         az = __builtin_fmaf(fm, dz, az);
                     }
-                    s12a += (double)s12s;
+                    ga += 24.0 * (double)gs;
                     s6a += (double)s6s;
                     r2a += (double)r2s;
                 } else {
@@ -1125,7 +1152,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                     while (mask) {
                         const int bit = __builtin_ctzll(mask);
                         mask &= mask - 1;
-                        const float4 d = top[-bit];
+                        const float4 d = rowp[bit];
                         const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         const float nrm = sqrtf(r2);
@@ -1141,10 +1168,11 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 wave_lds_sync();
             }
             if (LAW == LAW_LJ) {
-                // V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u, W = 48 s^12 - 24 s^6 - 2 beta u
+                // V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u = g / 12 - 2 s^6 + ...,
+                // W = 48 s^12 - 24 s^6 - 2 beta u = g - 2 beta u
                 const double u_sum = r2a * S.inv_rc2s;
-                pe = 4.0 * s12a - 4.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum;
-                w = 48.0 * s12a - 24.0 * s6a - 2.0 * S.beta * u_sum;
+                pe = ga * (1.0 / 12.0) - 2.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum;
+                w = ga - 2.0 * S.beta * u_sum;
                 if (S.eps != 1.0) {
                     ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
                     pe *= S.eps; w *= S.eps;
@@ -1159,7 +1187,10 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                     ax += r.x; ay += r.y; az += r.z;
                     nct += __float_as_int(r.w);
                 }
-                const double fx = ax, fy = ay, fz = az;
+                // the LJ law accumulates F / 24 (exact rescaling in double)
+                constexpr double fscale = (LAW == LAW_LJ) ? 24.0 : 1.0;
+                const double fx = fscale * (double)ax, fy = fscale * (double)ay,
+                             fz = fscale * (double)az;
                 if (p.ncount) p.ncount[si] = nct;
                 p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
@@ -1181,7 +1212,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
         }
     }
     double out[4];
-    block_sum4_lds(wacc, reinterpret_cast<double*>(st[wid]), out);
+    block_sum4_lds(wacc, reinterpret_cast<double*>(W.st), out);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
