@@ -79,6 +79,7 @@ struct lj_engine {
     int nslot = 0, nblk = 0;
     int ncell_total = 0, ntile = 0;
     int ncblk = 0;                  // cell-space blocks (tiled pair kernel)
+    int nsblk = 0;                  // sort blocks (groups of sort_k cells per wave)
     int pair_variant = LJ_PAIR_CELL_TILED;
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
@@ -90,13 +91,17 @@ struct lj_engine {
     SysDev* d_sys = nullptr;
     int* d_blk_sys = nullptr;
     int* d_cblk_sys = nullptr;
+    int* d_sblk_sys = nullptr;
     double* d_pos[2] = {nullptr, nullptr};
     double* d_vel[2] = {nullptr, nullptr};
     double* d_disp[2] = {nullptr, nullptr};
     int* d_id[2] = {nullptr, nullptr};
     double* d_force = nullptr;
-    int *d_cell_of = nullptr, *d_slot_of = nullptr, *d_perm = nullptr, *d_scell = nullptr;
-    int* d_perm_id = nullptr;
+    int* d_scell = nullptr;
+    int* d_members = nullptr;    // [cell][cell_cap] first arrivals of each cell
+    int2* d_ovf = nullptr;       // (slot, cell) of later arrivals
+    int* d_novf = nullptr;       // overflow entries: [0] this key pass, [1] the last scanned one
+    int cell_cap = 0;            // member records per cell
     float4* d_lpos = nullptr;
     int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
     int* d_cell_xyz = nullptr;   // per cell: x | y << 10 | z << 20
@@ -137,6 +142,7 @@ struct lj_engine {
         p.sys = d_sys;
         p.blk_sys = d_blk_sys;
         p.cblk_sys = d_cblk_sys;
+        p.sblk_sys = d_sblk_sys;
         auto planes = [&](double* base) {
             return ljk::Planes{base, base + nslot, base + 2 * (size_t)nslot};
         };
@@ -149,14 +155,14 @@ struct lj_engine {
         p.disp_n = planes(d_disp[from ^ 1]);
         p.id = d_id[from];
         p.id_n = d_id[from ^ 1];
-        p.cell_of = d_cell_of;
-        p.slot_of = d_slot_of;
         p.count = d_count;
         p.count_init = d_count_init;
         p.start = d_start;
         p.tile_sum = d_tile;
-        p.perm = d_perm;
-        p.perm_id = d_perm_id;
+        p.members = d_members;
+        p.ovf = d_ovf;
+        p.novf = d_novf;
+        p.cell_cap = cell_cap;
         p.scell = d_scell;
         p.lpos = d_lpos;
         p.partial = d_partial;
@@ -275,8 +281,8 @@ int enqueue_finalize(lj_engine* e, int mode);
 
 // Cell build, optionally for the integrated positions (first half kick,
 // drift, wrap): the key pass evaluates the integrated position for the keys
-// only, and the sorted move into the other buffer set applies the same
-// integration while it moves each particle.  Between them: scan, scatter.
+// only (and records each cell's arrivals), and the sort's move into the other
+// buffer set applies the same integration.  Between them: the scan.
 // Leaves e->cur pointing at the sorted set.
 int enqueue_cells(lj_engine* e, bool integrate)
 {
@@ -313,16 +319,14 @@ int enqueue_cells(lj_engine* e, bool integrate)
              })))
             return rc;
     }
-    if ((rc = timed(e, 3, "scatter", [&] {
-             launch(e, ljk::k_scatter, gb, tb, p);
-         })))
-        return rc;
+    // sort blocks: waves of sort_k cells
     const bool mixed = e->precision == LJ_PRECISION_MIXED;
-    rc = timed(e, 4, "sort_reorder", [&] {
-        if (mixed && integrate) launch(e, ljk::k_sort_reorder<true, true>, gb, tb, p);
-        else if (mixed) launch(e, ljk::k_sort_reorder<true, false>, gb, tb, p);
-        else if (integrate) launch(e, ljk::k_sort_reorder<false, true>, gb, tb, p);
-        else launch(e, ljk::k_sort_reorder<false, false>, gb, tb, p);
+    const dim3 gc(e->nsblk);
+    rc = timed(e, 4, "sort", [&] {
+        if (mixed && integrate) launch(e, ljk::k_sort_cells<true, true>, gc, tb, p);
+        else if (mixed) launch(e, ljk::k_sort_cells<true, false>, gc, tb, p);
+        else if (integrate) launch(e, ljk::k_sort_cells<false, true>, gc, tb, p);
+        else launch(e, ljk::k_sort_cells<false, false>, gc, tb, p);
     });
     if (rc) return rc;
     e->cur ^= 1;
@@ -603,7 +607,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             return fail(LJ_E_INVALID, "all systems of an engine must use the same force law");
         }
 
-    int atom_off = 0, cell_off = 0, blk = 0, cblk = 0;
+    int atom_off = 0, cell_off = 0, blk = 0, cblk = 0, sblk = 0;
     std::vector<int> count_init;
     for (int k = 0; k < nsys; ++k) {
         const lj_system_desc& d = systems[k];
@@ -681,6 +685,13 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         s.cblk_first = cblk;
         s.ncblk = std::max(1, (s.ncell + ljk::kCellsPerBlock - 1) / ljk::kCellsPerBlock);
         cblk += s.ncblk;
+        // sort: about 44 particles per wave (64 lanes), so that a wave's
+        // cells rarely hold more than 64
+        const double occ = s.ncell > 0 ? (double)s.n / s.ncell : 1.0;
+        s.sort_k = (int)std::min(63.0, std::max(1.0, std::floor(44.0 / occ)));
+        s.sblk_first = sblk;
+        const int waves = std::max(1, (s.ncell + s.sort_k - 1) / s.sort_k);
+        sblk += (waves + kBlock / 64 - 1) / (kBlock / 64);
         if (e->has_cells) {
             s.cell_off = cell_off;
             count_init.resize(cell_off + s.ncell + 1, 0);
@@ -695,13 +706,16 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     for (const SysDev& s : e->sys) e->n_tiled += s.tiled;
     e->nblk = blk;
     e->ncblk = cblk;
+    e->nsblk = sblk;
     e->ncell_total = cell_off;
     e->ntile = std::max(1, (cell_off + kScanTile - 1) / kScanTile);
 
-    std::vector<int> blk_sys(blk), cblk_sys(cblk);
+    std::vector<int> blk_sys(blk), cblk_sys(cblk), sblk_sys(sblk);
     for (int k = 0; k < nsys; ++k) {
         for (int b = 0; b < e->sys[k].nblk; ++b) blk_sys[e->sys[k].blk_first + b] = k;
         for (int b = 0; b < e->sys[k].ncblk; ++b) cblk_sys[e->sys[k].cblk_first + b] = k;
+        const int last = (k + 1 < nsys) ? e->sys[k + 1].sblk_first : sblk;
+        for (int b = e->sys[k].sblk_first; b < last; ++b) sblk_sys[b] = k;
     }
 
     auto cleanup_fail = [&](int rc) {
@@ -715,7 +729,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         return cleanup_fail(fail(LJ_E_HIP, "hipEventCreate failed"));
     const size_t ns = e->nslot;
     if ((rc = dev_alloc(&e->d_sys, nsys)) || (rc = dev_alloc(&e->d_blk_sys, blk)) ||
-        (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_force, 3 * ns)) ||
+        (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_sblk_sys, sblk)) ||
+        (rc = dev_alloc(&e->d_force, 3 * ns)) ||
         (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
         (rc = dev_alloc(&e->d_partial_c, 4 * (size_t)cblk)) ||
         (rc = dev_alloc(&e->d_cblk_list, (size_t)cblk)) || (rc = dev_alloc(&e->d_blk_list, (size_t)blk)) ||
@@ -731,8 +746,14 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             return cleanup_fail(rc);
     }
     if (e->has_cells) {
-        if ((rc = dev_alloc(&e->d_cell_of, ns)) || (rc = dev_alloc(&e->d_slot_of, ns)) ||
-            (rc = dev_alloc(&e->d_perm, ns)) || (rc = dev_alloc(&e->d_perm_id, ns)) ||
+        // Member records: 64 per cell unless the grid is much larger than the
+        // particle count (dilute systems); later arrivals of a cell go to the
+        // overflow list, which holds one entry per slot at most.
+        const size_t budget = std::max<size_t>((size_t)64 * ns, (size_t)1 << 24);
+        e->cell_cap = (int)std::max<size_t>(
+            1, std::min<size_t>(64, budget / std::max(1, e->ncell_total)));
+        if ((rc = dev_alloc(&e->d_members, (size_t)e->ncell_total * e->cell_cap)) ||
+            (rc = dev_alloc(&e->d_ovf, ns)) || (rc = dev_alloc(&e->d_novf, 2)) ||
             (rc = dev_alloc(&e->d_scell, ns)) ||
             (rc = dev_alloc(&e->d_count, e->ncell_total)) ||
             (rc = dev_alloc(&e->d_count_init, e->ncell_total)) ||
@@ -755,6 +776,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     bool ok = h2d(e->d_sys, e->sys.data(), nsys * sizeof(SysDev)) &&
               h2d(e->d_blk_sys, blk_sys.data(), blk * sizeof(int)) &&
               h2d(e->d_cblk_sys, cblk_sys.data(), cblk * sizeof(int)) &&
+              h2d(e->d_sblk_sys, sblk_sys.data(), sblk * sizeof(int)) &&
               h2d(e->d_force, zeros.data(), 3 * ns * sizeof(double)) &&
               h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
@@ -773,7 +795,12 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
                 const int cx = lc / (sd.nc[2] * sd.nc[1]);
                 xyz[sd.cell_off + lc] = cx | (cy << 10) | (cz << 20);
             }
-        ok = h2d(e->d_cell_xyz, xyz.data(), e->ncell_total * sizeof(int)) &&
+        const int novf0[2] = {0, 0};
+        // member records start as slot 0 (only records of the current pass are read)
+        ok = hipMemset(e->d_members, 0, (size_t)e->ncell_total * e->cell_cap * sizeof(int)) ==
+                 hipSuccess &&
+             h2d(e->d_novf, novf0, sizeof(novf0)) &&
+             h2d(e->d_cell_xyz, xyz.data(), e->ncell_total * sizeof(int)) &&
              h2d(e->d_count, count_init.data(), e->ncell_total * sizeof(int)) &&
              h2d(e->d_count_init, count_init.data(), e->ncell_total * sizeof(int));
     }
@@ -789,8 +816,7 @@ int lj_engine_destroy(lj_engine* e)
     if (!e) return LJ_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
-    void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_force, e->d_cell_of, e->d_slot_of, e->d_perm,
-                    e->d_perm_id,
+    void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_sblk_sys, e->d_force, e->d_members, e->d_ovf, e->d_novf,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
                     e->d_partial, e->d_partial_c, e->d_cblk_list, e->d_blk_list, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
                     e->d_measure, e->d_rdf};

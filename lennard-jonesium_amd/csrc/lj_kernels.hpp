@@ -51,6 +51,8 @@ struct SysDev {
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
     int tiled;         // pair forces from the tiled kernel (1) or the per-particle kernel (0)
+    int sort_k;        // cells per wave in the sort (about 44 particles per wave)
+    int sblk_first;    // first sort block (kBlock / 64 waves of sort_k cells each)
     double L[3];       // box
     double q[3];       // nc / L (the reference forms this quotient first)
     double a[3];       // L / nc (cell side, used for cell-local coordinates)
@@ -194,18 +196,19 @@ struct StepArgs {
     const SysDev* sys;
     const int* blk_sys;
     const int* cblk_sys;                // cell-space block -> system
+    const int* sblk_sys;                // sort block -> system
     Planes pos, vel, disp, force;       // current buffers
     Planes pos_n, vel_n, disp_n;        // next buffers (sort destination)
     int* id;                            // current ids
     int* id_n;                          // next ids
-    int* cell_of;                       // per slot (pre-sort order)
-    int* slot_of;                       // rank of arrival inside the cell
     int* count;                         // per cell, incl. tail cells
     const int* count_init;             // tail cells hold their padding count
     int* start;                         // exclusive scan, ncell_total + 1
     int* tile_sum;
-    int* perm;                          // arrival-ordered slot lists
-    int* perm_id;                       // particle id of each perm entry
+    int* members;                       // [cell][cell_cap]: slots of a cell's first arrivals
+    int2* ovf;                          // (slot, cell) of the later arrivals (nslot entries)
+    int* novf;                          // [0] overflow entries of this pass, [1] of the last scanned pass
+    int cell_cap;                       // member records per cell
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per particle-space block: sum v^2, PE, W, sum d^2
@@ -275,6 +278,19 @@ __device__ __forceinline__ void advance(const SysDev& S, double& x, double& y, d
     }
 }
 
+// The g-th arrival (0-based) of slot s in cell c: the first cell_cap of a
+// cell go to its member record, later ones to the overflow list.
+__device__ __forceinline__ void record_member(const StepArgs& p, int s, int c, int g)
+{
+    if (g < p.cell_cap) {
+        p.members[(size_t)c * p.cell_cap + g] = s;
+    } else {
+        // at most one entry per slot and pass (the scan empties the list)
+        const int o = atomicAdd(p.novf, 1);
+        if (o < p.nslot) p.ovf[o] = make_int2(s, c);
+    }
+}
+
 // MODE 0: keys of the current positions; 1: integrate in place (and key);
 // 2: key of the integrated position only -- the sort applies the step while
 // it moves the particle, so the state is read once and written once.
@@ -337,10 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
                 if (n) hist[t] = atomicAdd(&p.count[kmin + t], n);
             }
             __syncthreads();
-            if (c >= 0) {
-                p.cell_of[s] = c;
-                p.slot_of[s] = hist[c - kmin] + r;
-            }
+            if (c >= 0) record_member(p, s, c, hist[c - kmin] + r);
             return;
         }
         // Wide windows (slots not yet in cell order): one atomic per run of
@@ -357,10 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
         int base = 0;
         if (head && c >= 0) base = atomicAdd(&p.count[c], end - lane);
         base = __shfl(base, h, 64);
-        if (c >= 0) {
-            p.cell_of[s] = c;
-            p.slot_of[s] = base + (lane - h);
-        }
+        if (c >= 0) record_member(p, s, c, base + (lane - h));
     }
 }
 
@@ -440,6 +450,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p, FinalArgs f)
         return;
     }
     __shared__ int s_off;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the key pass's overflow count, for the sort; the next pass starts empty
+        p.novf[1] = p.novf[0];
+        p.novf[0] = 0;
+    }
     // offset of this tile: sum of the preceding tile sums
     int acc = 0;
     for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) acc += p.tile_sum[t];
@@ -470,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p, FinalArgs f)
 // Single-block form for up to kScanOneMax * 4096 cells: 1024 threads load
 // all their int4 tiles first (one memory latency), scan every tile's waves
 // side by side, and one wave combines the wave sums into offsets: two block
-// barriers in all.  The counts are zeroed later, by k_sort_reorder.
+// barriers in all.  The counts are zeroed later, by k_sort_cells.
 // Blocks 1.. of the launch finish the previous step's records instead
 // (section 7), in parallel with the scan.
 constexpr int kScanOneThreads = 1024;
@@ -519,7 +534,12 @@ __global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalA
             if (lane < NW) wsum[t][lane] = carry + xi - x;
             carry += tile_tot;
         }
-        if (lane == 0) p.start[n] = carry;
+        if (lane == 0) {
+            p.start[n] = carry;
+            // the key pass's overflow count, for the sort; the next pass starts empty
+            p.novf[1] = p.novf[0];
+            p.novf[0] = 0;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -540,59 +560,22 @@ __global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalA
 }
 
 // ---------------------------------------------------------------------------
-// 4. scatter: slot lists per cell in arrival order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs p)
-{
-    const int b = blockIdx.x;
-    const SysDev& S = p.sys[p.blk_sys[b]];
-    const int s = b * kBlock + threadIdx.x;
-    if (s - S.atom_off >= S.n) return;
-    const int q = p.start[p.cell_of[s]] + p.slot_of[s];
-    p.perm[q] = s;
-    p.perm_id[q] = p.id[s];
-}
-
-// ---------------------------------------------------------------------------
-// 5. order each cell by particle id and move the particles into their
-//    sorted slots (next buffers); mixed precision also writes cell-local
-//    float coordinates.
+// 4. sort: one wave per cell orders the cell's members by particle id and
+//    moves each into its sorted slot start[c] + rank (next buffers), with the
+//    first half kick, drift and wrap when integrating; mixed precision also
+//    writes cell-local float coordinates.  The sorted order is a function of
+//    the positions alone, whatever order the key pass's atomics took.
 // ---------------------------------------------------------------------------
 template <bool MIXED_, bool INTEGRATE>
-__global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
+__device__ __forceinline__ void move_particle(const StepArgs& p, const SysDev& S, int c, int cx,
+                                              int cy, int cz, int s, int me, int dst)
 {
-    const int b = blockIdx.x;
-    const SysDev& S = p.sys[p.blk_sys[b]];
-    const int u = b * kBlock + threadIdx.x;
-    if (u - S.atom_off >= S.n) return;
-    const int old = p.perm[u];
-    const int me = p.perm_id[u];
-    const int c = p.cell_of[old];
-    // The particle's state is loaded before the rank search so that its
-    // latency overlaps the search (every slot has a force, zero before the
-    // first calculation).
-    double x = p.pos.x[old], y = p.pos.y[old], z = p.pos.z[old];
-    double vx = p.vel.x[old], vy = p.vel.y[old], vz = p.vel.z[old];
-    double dx = p.disp.x[old], dy = p.disp.y[old], dz = p.disp.z[old];
-    double fx = 0., fy = 0., fz = 0.;
-    if (INTEGRATE) { fx = p.force.x[old]; fy = p.force.y[old]; fz = p.force.z[old]; }
-    const int c0 = p.start[c], c1 = p.start[c + 1];
-    int cx = 0, cy = 0, cz = 0;
-    if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
-    // rank by id inside the cell: eight loads in flight per round (indices
-    // past the cell's end re-read its last entry and are not counted)
-    int rank = 0;
-    for (int q = c0; q < c1; q += 8) {
-        int v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = p.perm_id[min(q + k, c1 - 1)];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) rank += (q + k < c1 && v[k] < me) ? 1 : 0;
-    }
-    const int dst = c0 + rank;
+    double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+    double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+    double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
     if (INTEGRATE && S.active) {
         double ix, iy, iz;
-        advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+        advance(S, x, y, z, vx, vy, vz, p.force.x[s], p.force.y[s], p.force.z[s], ix, iy, iz);
         dx = dx + ix; dy = dy + iy; dz = dz + iz;
     }
     float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -605,9 +588,150 @@ __global__ __launch_bounds__(kBlock) void k_sort_reorder(StepArgs p)
     p.id_n[dst] = me;
     p.scell[dst] = c;
     if (MIXED_) p.lpos[dst] = lp;
+}
+
+// One cell (local index lc) of n members from slot c0, one wave: the path for
+// cells that do not fit a shared wave.  Lane j holds the j-th arrival; cells
+// with more members than records take the rest from the overflow list.
+template <bool MIXED_, bool INTEGRATE>
+__device__ __forceinline__ void sort_one_cell(const StepArgs& p, const SysDev& S, int lc, int c0,
+                                              int n)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = S.cell_off + lc;
+    const int cz = lc % S.nc[2], cxy = lc / S.nc[2];
+    const int cy = cxy % S.nc[1], cx = cxy / S.nc[1];
+    const int cap = p.cell_cap;
+    const int* mem = p.members + (size_t)c * cap;
+    if (n <= cap) {
+        const bool act = lane < n;
+        const int s = mem[act ? lane : 0];
+        const int me = act ? p.id[s] : 0x7fffffff;
+        int rank = 0;
+        for (int j = 0; j < n; ++j) rank += (__builtin_amdgcn_readlane(me, j) < me) ? 1 : 0;
+        if (act) move_particle<MIXED_, INTEGRATE>(p, S, c, cx, cy, cz, s, me, c0 + rank);
+        return;
+    }
+    // more members than records: the others are in the overflow list of the
+    // pass (unordered), searched once per 64 entries for each group of
+    // members whose ranks are wanted
+    const int novf = __builtin_amdgcn_readfirstlane(min(p.novf[1], p.nslot));
+    const bool actA = lane < cap;
+    const int sA = mem[actA ? lane : 0];
+    const int idA = actA ? p.id[sA] : 0x7fffffff;
+    auto rank_of = [&](int v) {   // members of c with an id below v
+        int r = 0;
+        for (int j = 0; j < cap; ++j) r += (__builtin_amdgcn_readlane(idA, j) < v) ? 1 : 0;
+        for (int w0 = 0; w0 < novf; w0 += 64) {
+            const int o = w0 + lane;
+            int idB = 0x7fffffff;
+            if (o < novf) {
+                const int2 en = p.ovf[o];
+                if (en.y == c) idB = p.id[en.x];
+            }
+            const int m = min(64, novf - w0);
+            for (int j = 0; j < m; ++j) r += (__builtin_amdgcn_readlane(idB, j) < v) ? 1 : 0;
+        }
+        return r;
+    };
+    const int rA = rank_of(idA);
+    if (actA) move_particle<MIXED_, INTEGRATE>(p, S, c, cx, cy, cz, sA, idA, c0 + rA);
+    for (int w0 = 0; w0 < novf; w0 += 64) {
+        const int o = w0 + lane;
+        int sB = 0, idB = 0x7fffffff;
+        bool mine = false;
+        if (o < novf) {
+            const int2 en = p.ovf[o];
+            if (en.y == c) { mine = true; sB = en.x; idB = p.id[sB]; }
+        }
+        if (__ballot(mine) == 0ull) continue;
+        const int rB = rank_of(idB);
+        if (mine) move_particle<MIXED_, INTEGRATE>(p, S, c, cx, cy, cz, sB, idB, c0 + rB);
+    }
+}
+
+// A wave takes sort_k consecutive cells.  When their members fit 64 lanes
+// and every one is recorded, lane l holds member l - off(k) of the group's
+// cell k (off: the cell's start relative to the group's), and ranks by id
+// are counted among the lanes of the same cell; otherwise the wave walks the
+// cells one by one.
+template <bool MIXED_, bool INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
+{
+    const int b = blockIdx.x;
+    const SysDev& S = p.sys[p.sblk_sys[b]];
+    const int K = S.sort_k;
+    const int lane = threadIdx.x & 63;
+    const int wv = (b - S.sblk_first) * (kBlock / 64) +
+                   __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lc0 = wv * K;
+    if (lc0 >= S.ncell) return;
+    const int ng = min(K, S.ncell - lc0);    // cells of this wave (at most 63)
+    const int cf = S.cell_off + lc0;
+    // lane k <= ng: start of the group's cell k (k = ng: the group's end)
+    const int st = p.start[cf + min(lane, ng)];
+    const int g0 = __builtin_amdgcn_readfirstlane(st);
+    const int total = __builtin_amdgcn_readlane(st, ng) - g0;
+    if (total == 0) return;   // the counts are zero already
+    // the shuffle runs on every lane (a lane reading an inactive one gets 0)
+    const int nxt = __shfl_down(st, 1, 64);
+    const int cnt = (lane < ng) ? nxt - st : 0;
+    const int maxcnt = __builtin_amdgcn_readfirstlane(wave_extreme<true>(cnt));
+    if (total <= 64 && maxcnt <= p.cell_cap) {
+        // my cell kk and its members' lanes [jb, je)
+        int kk = 0, jb = 0, je = total;
+        for (int k = 0; k < ng; ++k) {
+            const int o = __builtin_amdgcn_readlane(st, k) - g0;
+            if (o <= lane) { kk = k; jb = o; } else { je = min(je, o); }
+        }
+        const bool act = lane < total;
+        const int c = cf + kk;
+        const int rec = p.members[(size_t)c * p.cell_cap + (act ? lane - jb : 0)];
+        const int s = act ? rec : S.atom_off;   // idle lanes read a valid slot
+        const int me = act ? p.id[s] : 0x7fffffff;
+        double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+        double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+        double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
+        double fx = 0., fy = 0., fz = 0.;
+        if (INTEGRATE) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
+        int cx = 0, cy = 0, cz = 0;
+        if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+        // rank: members of my cell (lanes jb .. je - 1) with a smaller id
+        // (maxcnt is uniform, so every lane runs every shuffle)
+        const int m = je - jb;
+        int rank = 0;
+        for (int t = 0; t < maxcnt; ++t) {
+            const int v = __shfl(me, jb + t, 64);
+            rank += (t < m && v < me) ? 1 : 0;
+        }
+        if (act) {
+            const int dst = g0 + jb + rank;
+            if (INTEGRATE && S.active) {
+                double ix, iy, iz;
+                advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+                dx = dx + ix; dy = dy + iy; dz = dz + iz;
+            }
+            float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (MIXED_)
+                lp = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                                 (float)(z - cz * S.a[2]), 0.0f);
+            p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
+            p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
+            p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
+            p.id_n[dst] = me;
+            p.scell[dst] = c;
+            if (MIXED_) p.lpos[dst] = lp;
+        }
+    } else {
+        for (int k = 0; k < ng; ++k) {
+            const int c0 = __builtin_amdgcn_readlane(st, k);
+            const int n = __builtin_amdgcn_readlane(st, k + 1) - c0;
+            if (n) sort_one_cell<MIXED_, INTEGRATE>(p, S, lc0 + k, c0, n);
+        }
+    }
     // ready for the next counting pass (tail cells keep theirs); stored last,
     // after every read of the system's parameters
-    if (u == c0) p.count[c] = 0;
+    if (cnt > 0) p.count[cf + lane] = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -476,3 +476,30 @@ def test_large_cell_grid_records_track_oracle():
     err = np.abs(runs[0] - ref)
     _say("large grid max abs err", err.max(), "max |ref|", np.abs(ref).max())
     assert np.all(err <= 1e-10 * np.abs(ref) + 1e-12)   # PE and W are ~0 in this gas
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_crowded_cells_track_oracle(precision):
+    """Cutoff 3.9 sigma on a 500-particle liquid (L = 8.40): a 2^3 cell grid of
+    about 62 particles per cell, so cells often receive more arrivals than
+    their 64 member records and the sort ranks the rest from the overflow
+    list.  After 30 steps the cell list equals the oracle's for the engine's
+    own positions (bit-exact), and the records track the oracle's run."""
+    eng = _engine_mod()
+    st, L = ljref.initial_condition(500, 0.8442, 0.72, seed=5)
+    assert int(L // 3.9) == 2
+    ref_st = st.copy()
+    ref = ljref.run(ref_st, L, 0.005, 30, rc=3.9)
+    e = eng.Engine([eng.SystemSpec(500, L, cutoff=3.9)], precision=precision)
+    e.set_state(0, st.pos, st.vel)
+    rec = e.step(30)[:, 0, :]
+    start, atoms = e.cell_list(0)
+    pos = e.get_state(0)["pos"]
+    e.close()
+    rstart, ratoms = ljref.cell_list(pos, L, 3.9)
+    _say("crowded cells", precision, np.diff(start))
+    assert np.diff(start).max() > 64          # the overflow path ran
+    assert np.array_equal(start, rstart) and np.array_equal(atoms, ratoms)
+    err = np.abs(rec - ref) / np.abs(ref)
+    _say("crowded cells max rel err", err.max())
+    assert err.max() < (1e-11 if precision == "fp64" else 1e-5)
