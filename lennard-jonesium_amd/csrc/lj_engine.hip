@@ -61,7 +61,7 @@ struct DevBuf {
     }
 };
 
-constexpr int kMaxParts = 64;   // first-level parts per system of a large grid's record sums
+constexpr int kMaxParts = 64;   // first-level parts per system of the record sums (one wave)
 
 struct KernelTimer {
     const char* name;
@@ -198,16 +198,16 @@ struct lj_engine {
         f.cell_blocks = (mode == 0 ? sys[0].law != ljk::LAW_NONE : last_pair_force) ? 1 : 0;
         f.pending = d_pending;
         f.partial2 = d_partial2;
-        // first-level parts: about 1024 block sums each, at most kMaxParts
+        // first-level parts: about one block sum per thread, at most kMaxParts
         int nbmax = 1;
         for (const SysDev& s : sys)
             nbmax = std::max(nbmax, (f.cell_blocks && s.tiled) ? s.ncblk : s.nblk);
-        f.nf = std::min(kMaxParts, (nbmax + 1023) / 1024);
+        f.nf = std::min(kMaxParts, (nbmax + kBlock - 1) / kBlock);
         return f;
     }
 
-    // Grids above the single-block scan's reach sum records in two levels.
-    bool two_level_records() const
+    // Grids above the single-block scan's reach scan in two launches.
+    bool two_level_scan() const
     {
         return has_cells && ncell_total > ljk::kScanOneMax * 4 * ljk::kScanOneThreads;
     }
@@ -288,34 +288,34 @@ int enqueue_cells(lj_engine* e, bool integrate)
 {
     const ljk::StepArgs p = e->args(e->cur);
     const dim3 gb(e->nblk), tb(kBlock);
+    // a step also finishes the previous step's records (pending flags
+    // decide): first-level sums in blocks of the key pass, the second level
+    // in blocks of the scan
+    ljk::FinalArgs f0 = e->fargs(0);
+    const int nfin = integrate ? e->nsys * f0.nf : 0;
     int rc = LJ_OK;
     if (integrate)
         rc = timed(e, 0, "integrate_key", [&] {
-            launch(e, ljk::k_integrate_key<2, true>, gb, tb, p);
+            launch(e, ljk::k_integrate_key<2, true>, dim3(e->nblk + nfin), tb, p, f0, nfin);
         });
     else
         rc = timed(e, 0, "integrate_key", [&] {
-            launch(e, ljk::k_integrate_key<0, true>, gb, tb, p);
+            launch(e, ljk::k_integrate_key<0, true>, gb, tb, p, f0, 0);
         });
     if (rc) return rc;
-    if (!e->two_level_records()) {
-        // a step also finishes the previous step's records (pending flags decide)
-        const ljk::FinalArgs f = e->fargs(0);
-        const int nfin = integrate ? e->nsys : 0;
+    const int nred = integrate ? e->nsys : 0;
+    if (!e->two_level_scan()) {
         if ((rc = timed(e, 1, "scan", [&] {
-                 launch(e, ljk::k_scan_one, dim3(1 + nfin), dim3(ljk::kScanOneThreads), p, f);
+                 launch(e, ljk::k_scan_one, dim3(1 + nred), dim3(ljk::kScanOneThreads), p, f0);
              })))
             return rc;
     } else {
-        // the previous step's records in two levels inside the two scan launches
-        ljk::FinalArgs f = e->fargs(0);
-        if (!integrate) f.nsys = 0;
         if ((rc = timed(e, 1, "scan_tiles", [&] {
-                 launch(e, ljk::k_scan_tiles, dim3(e->ntile + f.nsys * f.nf), tb, p, f);
+                 launch(e, ljk::k_scan_tiles, dim3(e->ntile), tb, p);
              })))
             return rc;
         if ((rc = timed(e, 2, "scan_apply", [&] {
-                 launch(e, ljk::k_scan_apply, dim3(e->ntile + f.nsys), tb, p, f);
+                 launch(e, ljk::k_scan_apply, dim3(e->ntile + nred), tb, p, f0);
              })))
             return rc;
     }
@@ -418,12 +418,8 @@ int enqueue_finalize(lj_engine* e, int mode)
 {
     const ljk::FinalArgs f = e->fargs(mode);
     return timed(e, 6, "finalize", [&] {
-        if (e->two_level_records()) {
-            launch(e, ljk::k_finalize_parts, dim3(e->nsys * f.nf), dim3(kBlock), f);
-            launch(e, ljk::k_finalize_reduced, dim3(e->nsys), dim3(kBlock), f);
-        } else {
-            launch(e, ljk::k_finalize, dim3(e->nsys), dim3(ljk::kFinalThreads), f);
-        }
+        launch(e, ljk::k_finalize_parts, dim3(e->nsys * f.nf), dim3(kBlock), f);
+        launch(e, ljk::k_finalize_reduced, dim3(e->nsys), dim3(kBlock), f);
     });
 }
 
@@ -440,7 +436,8 @@ int enqueue_step(lj_engine* e)
         const ljk::StepArgs p = e->args(e->cur);
         if ((rc = enqueue_finalize(e, 0))) return rc;
         if ((rc = timed(e, 0, "integrate_key", [&] {
-                 launch(e, ljk::k_integrate_key<1, false>, dim3(e->nblk), dim3(kBlock), p);
+                 launch(e, ljk::k_integrate_key<1, false>, dim3(e->nblk), dim3(kBlock), p,
+                        e->fargs(0), 0);
              })))
             return rc;
     }

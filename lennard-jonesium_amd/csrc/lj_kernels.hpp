@@ -242,12 +242,10 @@ struct FinalArgs {
     int mode;
     int cell_blocks;   // 1: a system with tiled pair forces has cell-space sums; 0: all particle-space
     int* pending;      // per system: set by a step's pair kernel, cleared here
-    double* partial2;  // [nsys][nf] first-level sums of large grids (see k_scan_tiles)
+    double* partial2;  // [nsys][nf] first-level sums (see finalize_part)
     int nf;            // first-level parts per system
 };
 
-template <int NT>
-__device__ void finalize_system(const FinalArgs& f, int k);
 template <int NT>
 __device__ void finalize_range(const FinalArgs& f, int k, const double* part, int nb);
 __device__ void finalize_part(const FinalArgs& f, int k, int r);
@@ -294,10 +292,17 @@ __device__ __forceinline__ void record_member(const StepArgs& p, int s, int c, i
 // MODE 0: keys of the current positions; 1: integrate in place (and key);
 // 2: key of the integrated position only -- the sort applies the step while
 // it moves the particle, so the state is read once and written once.
+// The first nfin blocks compute the first level of the previous step's
+// record sums instead (f.nsys * f.nf parts, section 7); the scan's extra
+// blocks finish them.
 template <int MODE, bool KEYS>
-__global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p)
+__global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p, FinalArgs f, int nfin)
 {
-    const int b = blockIdx.x;
+    if ((int)blockIdx.x < nfin) {
+        finalize_part(f, blockIdx.x / f.nf, blockIdx.x % f.nf);
+        return;
+    }
+    const int b = blockIdx.x - nfin;
     const SysDev& S = p.sys[p.blk_sys[b]];
     const int s = b * kBlock + threadIdx.x;
     const bool active = s - S.atom_off < S.n;
@@ -420,17 +425,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total)
     return off + inc - v;
 }
 
-//    Blocks past the tiles compute the first level of the previous step's
-//    record sums (f.nsys * f.nf of them; section 7), k_scan_apply's extra
-//    blocks the second.
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p, FinalArgs f)
+//    k_scan_apply's blocks past the tiles finish the previous step's record
+//    sums (the second level; section 7).
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(StepArgs p)
 {
-    const int ntile = (p.ncell_total + kScanTile - 1) / kScanTile;
-    if ((int)blockIdx.x >= ntile) {
-        const int x = blockIdx.x - ntile;
-        finalize_part(f, x / f.nf, x % f.nf);
-        return;
-    }
     const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
     int v = 0;
 #pragma unroll
@@ -486,15 +484,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p, FinalArgs f)
 // all their int4 tiles first (one memory latency), scan every tile's waves
 // side by side, and one wave combines the wave sums into offsets: two block
 // barriers in all.  The counts are zeroed later, by k_sort_cells.
-// Blocks 1.. of the launch finish the previous step's records instead
-// (section 7), in parallel with the scan.
+// Blocks 1.. of the launch finish the previous step's records instead (the
+// second level of the record sums, one system each; section 7).
 constexpr int kScanOneThreads = 1024;
 constexpr int kScanOneMax = 8;
 
 __global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalArgs f)
 {
     if (blockIdx.x > 0) {
-        finalize_system<kScanOneThreads>(f, blockIdx.x - 1);
+        // at most 64 parts: all in wave 0, the same sums as a kBlock reduction
+        const int k = blockIdx.x - 1;
+        finalize_range<kScanOneThreads>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
         return;
     }
     constexpr int NW = kScanOneThreads / 64;
@@ -1733,16 +1733,8 @@ __device__ __forceinline__ const double* block_sums(const FinalArgs& f, const Sy
     return cells ? f.partial_c + 4 * (size_t)S.cblk_first : f.partial + 4 * (size_t)S.blk_first;
 }
 
-template <int NT>
-__device__ void finalize_system(const FinalArgs& f, int k)
-{
-    int nb;
-    const double* part = block_sums(f, f.sys[k], nb);
-    finalize_range<NT>(f, k, part, nb);
-}
-
-// First level for large grids: part r of f.nf contiguous parts of system k's
-// block sums, summed by one kBlock-thread block into f.partial2.
+// First level: part r of f.nf contiguous parts of system k's block sums,
+// summed by one kBlock-thread block into f.partial2.
 __device__ void finalize_part(const FinalArgs& f, int k, int r)
 {
     int nb;
@@ -1763,18 +1755,9 @@ __device__ void finalize_part(const FinalArgs& f, int k, int r)
     }
 }
 
-// One 1024-thread block per system (a large system has tens of thousands of
-// block sums: more threads keep more loads in flight).
-constexpr int kFinalThreads = 1024;
-
-__global__ __launch_bounds__(kFinalThreads) void k_finalize(FinalArgs f)
-{
-    finalize_system<kFinalThreads>(f, blockIdx.x);
-}
-
-// The two levels of k_scan_tiles / k_scan_apply as launches of their own, for
-// records finished outside a cell build on large grids (the same sums in the
-// same order, so a record does not depend on where a run was split).
+// The two levels of the key pass's and the scan's record blocks as launches
+// of their own, for records finished outside a cell build (the same sums in
+// the same order, so a record does not depend on where a run was split).
 __global__ __launch_bounds__(kBlock) void k_finalize_parts(FinalArgs f)
 {
     finalize_part(f, blockIdx.x / f.nf, blockIdx.x % f.nf);
