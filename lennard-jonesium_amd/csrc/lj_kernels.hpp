@@ -1412,7 +1412,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
         float4* sw = st[wid];
         const double rc2d = S.rc2;
         const double sig2 = S.sigma2;
-        const double cwd = -2.0 * S.beta * S.inv_rc2s;
+        const double cwd24 = -2.0 * S.beta * S.inv_rc2s / 24.0;
         const float rc2f = S.rc2f;
         const float keepf = rc2f + 2e-3f;   // bounding-box pruning, wider than the phase-1 margin
 
@@ -1508,7 +1508,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
             fetch(lane, nq, nx, ny, nz);
             int g0 = 0;
             double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, w = 0.0;
-            double s12a = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of s^12, s^6, r^2
+            double ga = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of g / 24, s^6, r^2
             int nct = 0;
             while (g0 < total) {
                 // ---- stage the surviving neighbours of the next candidates, 64 at a time
@@ -1593,16 +1593,17 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                     if (r2 < rc2d && r2 > 0.0) {
                         ++nct;
                         if (LAW == LAW_LJ) {
-                            // s = (sigma / r)^6; W = 48 s^2 - 24 s - 2 beta u, F = W sep / r^2
+                            // s = (sigma / r)^6; W = 48 s^2 - 24 s - 2 beta u, F = W sep / r^2,
+                            // accumulated as g / 24 = s (2 s - 1) and F / 24 = (g / 24) / r^2 +
+                            // cw / 24 (the 24 is restored per particle)
                             const double inv = rcp_f64(r2);
                             const double t = inv * sig2;
                             const double s6 = t * t * t;
-                            const double s12 = s6 * s6;
-                            s12a += s12;
+                            const double g = s6 * __builtin_fma(2.0, s6, -1.0);
+                            ga += g;
                             s6a += s6;
                             r2a += r2;
-                            const double ww = __builtin_fma(48.0, s12, __builtin_fma(-24.0, s6, cwd * r2));
-                            const double fm = ww * inv;
+                            const double fm = __builtin_fma(g, inv, cwd24);
                             fx = __builtin_fma(fm, dx, fx);
                             fy = __builtin_fma(fm, dy, fy);
                             fz = __builtin_fma(fm, dz, fz);
@@ -1614,12 +1615,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                 wave_lds_sync();
             }
             if (LAW == LAW_LJ) {
-                // V = 4 s^2 - 4 s + (alpha - beta) + beta u, W = 48 s^2 - 24 s - 2 beta u,
+                // V = 4 s^2 - 4 s + (alpha - beta) + beta u = 2 g' - 2 s + ... and
+                // W = 48 s^2 - 24 s - 2 beta u = 24 g' - 2 beta u (g' = g / 24),
                 // times epsilon (physics/lennard_jones_force.cpp:107-121)
                 const double u_sum = r2a * S.inv_rc2s;
-                pe = S.eps * (4.0 * s12a - 4.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
-                w = S.eps * (48.0 * s12a - 24.0 * s6a - 2.0 * S.beta * u_sum);
-                fx *= S.eps; fy *= S.eps; fz *= S.eps;
+                pe = S.eps * (2.0 * ga - 2.0 * s6a + (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
+                w = S.eps * (24.0 * ga - 2.0 * S.beta * u_sum);
+                const double fs = 24.0 * S.eps;
+                fx *= fs; fy *= fs; fz *= fs;
             }
             // ---- sum the slices of each particle in a fixed order
             sdw[4 * lane] = fx;
