@@ -650,11 +650,68 @@ __device__ __forceinline__ void sort_one_cell(const StepArgs& p, const SysDev& S
     }
 }
 
-// A wave takes sort_k consecutive cells.  When their members fit 64 lanes
-// and every one is recorded, lane l holds member l - off(k) of the group's
-// cell k (off: the cell's start relative to the group's), and ranks by id
-// are counted among the lanes of the same cell; otherwise the wave walks the
-// cells one by one.
+// Cells k0 .. k1 - 1 of a wave's group (their members fit 64 lanes and every
+// one is recorded): lane l holds member l - off(k) of cell k (off: the cell's
+// start relative to cell k0's), and ranks by id are counted among the lanes
+// of the same cell.  st: lane k holds the start of the group's cell k.
+template <bool MIXED_, bool INTEGRATE>
+__device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, int cf, int st,
+                                           int k0, int k1, int maxcnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int g0 = __builtin_amdgcn_readlane(st, k0);
+    const int total = __builtin_amdgcn_readlane(st, k1) - g0;
+    if (total == 0) return;
+    // my cell kk and its members' lanes [jb, je)
+    int kk = k0, jb = 0, je = total;
+    for (int k = k0; k < k1; ++k) {
+        const int o = __builtin_amdgcn_readlane(st, k) - g0;
+        if (o <= lane) { kk = k; jb = o; } else { je = min(je, o); }
+    }
+    const bool act = lane < total;
+    const int c = cf + kk;
+    const int rec = p.members[(size_t)c * p.cell_cap + (act ? lane - jb : 0)];
+    const int s = act ? rec : S.atom_off;   // idle lanes read a valid slot
+    const int me = act ? p.id[s] : 0x7fffffff;
+    double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+    double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+    double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
+    double fx = 0., fy = 0., fz = 0.;
+    if (INTEGRATE) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
+    int cx = 0, cy = 0, cz = 0;
+    if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+    // rank: members of my cell (lanes jb .. je - 1) with a smaller id
+    // (maxcnt is uniform, so every lane runs every shuffle)
+    const int m = je - jb;
+    int rank = 0;
+    for (int t = 0; t < maxcnt; ++t) {
+        const int v = __shfl(me, jb + t, 64);
+        rank += (t < m && v < me) ? 1 : 0;
+    }
+    if (act) {
+        const int dst = g0 + jb + rank;
+        if (INTEGRATE && S.active) {
+            double ix, iy, iz;
+            advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+            dx = dx + ix; dy = dy + iy; dz = dz + iz;
+        }
+        float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MIXED_)
+            lp = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                             (float)(z - cz * S.a[2]), 0.0f);
+        p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
+        p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
+        p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
+        p.id_n[dst] = me;
+        p.scell[dst] = c;
+        if (MIXED_) p.lpos[dst] = lp;
+    }
+}
+
+// A wave takes sort_k consecutive cells.  Runs of consecutive cells whose
+// members fit 64 lanes are sorted together (sort_group; normally the whole
+// group in one pass); when a cell has more members than records, the wave
+// walks the cells one by one (sort_one_cell).
 template <bool MIXED_, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 {
@@ -670,57 +727,21 @@ __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
     const int cf = S.cell_off + lc0;
     // lane k <= ng: start of the group's cell k (k = ng: the group's end)
     const int st = p.start[cf + min(lane, ng)];
-    const int g0 = __builtin_amdgcn_readfirstlane(st);
-    const int total = __builtin_amdgcn_readlane(st, ng) - g0;
+    const int total = __builtin_amdgcn_readlane(st, ng) - __builtin_amdgcn_readfirstlane(st);
     if (total == 0) return;   // the counts are zero already
     // the shuffle runs on every lane (a lane reading an inactive one gets 0)
     const int nxt = __shfl_down(st, 1, 64);
     const int cnt = (lane < ng) ? nxt - st : 0;
     const int maxcnt = __builtin_amdgcn_readfirstlane(wave_extreme<true>(cnt));
-    if (total <= 64 && maxcnt <= p.cell_cap) {
-        // my cell kk and its members' lanes [jb, je)
-        int kk = 0, jb = 0, je = total;
-        for (int k = 0; k < ng; ++k) {
-            const int o = __builtin_amdgcn_readlane(st, k) - g0;
-            if (o <= lane) { kk = k; jb = o; } else { je = min(je, o); }
-        }
-        const bool act = lane < total;
-        const int c = cf + kk;
-        const int rec = p.members[(size_t)c * p.cell_cap + (act ? lane - jb : 0)];
-        const int s = act ? rec : S.atom_off;   // idle lanes read a valid slot
-        const int me = act ? p.id[s] : 0x7fffffff;
-        double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
-        double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
-        double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
-        double fx = 0., fy = 0., fz = 0.;
-        if (INTEGRATE) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
-        int cx = 0, cy = 0, cz = 0;
-        if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
-        // rank: members of my cell (lanes jb .. je - 1) with a smaller id
-        // (maxcnt is uniform, so every lane runs every shuffle)
-        const int m = je - jb;
-        int rank = 0;
-        for (int t = 0; t < maxcnt; ++t) {
-            const int v = __shfl(me, jb + t, 64);
-            rank += (t < m && v < me) ? 1 : 0;
-        }
-        if (act) {
-            const int dst = g0 + jb + rank;
-            if (INTEGRATE && S.active) {
-                double ix, iy, iz;
-                advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
-                dx = dx + ix; dy = dy + iy; dz = dz + iz;
-            }
-            float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (MIXED_)
-                lp = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
-                                 (float)(z - cz * S.a[2]), 0.0f);
-            p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
-            p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
-            p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
-            p.id_n[dst] = me;
-            p.scell[dst] = c;
-            if (MIXED_) p.lpos[dst] = lp;
+    if (maxcnt <= p.cell_cap) {
+        // passes over runs of cells [k0, k1) with at most 64 members (one
+        // cell has at most cell_cap <= 64, so every pass takes a cell)
+        for (int k0 = 0; k0 < ng;) {
+            const int base = __builtin_amdgcn_readlane(st, k0);
+            int k1 = k0 + 1;
+            while (k1 < ng && __builtin_amdgcn_readlane(st, k1 + 1) - base <= 64) ++k1;
+            sort_group<MIXED_, INTEGRATE>(p, S, cf, st, k0, k1, maxcnt);
+            k0 = k1;
         }
     } else {
         for (int k = 0; k < ng; ++k) {

@@ -479,6 +479,39 @@ def test_large_cell_grid_records_track_oracle():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sort_groups_split_in_passes(precision):
+    """A slab of 4096 particles (simple cubic, spacing 1.25) in two cell layers
+    of a 16^3 grid: the mean occupancy (1 per cell) gives each sort wave 44
+    cells, which here hold ~350 particles at 8 per cell, so every occupied
+    group is sorted in several 64-lane passes.  Cell lists equal the oracle's
+    (bit-exact) and 20 steps track the oracle's records."""
+    eng = _engine_mod()
+    L = 40.0
+    g = np.arange(32) * 1.25 + 0.3
+    ii, jj, kk = np.meshgrid(np.arange(4) * 1.25 + 0.3, g, g, indexing="ij")
+    pos = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
+    rng = np.random.default_rng(4)
+    vel = rng.normal(0.0, 0.3, size=pos.shape)
+    n = len(pos)
+    st = ljref.State(n, pos.copy(), vel.copy())
+    ref = ljref.run(st, L, 0.005, 20)
+    e = eng.Engine([eng.SystemSpec(n, L)], precision=precision)
+    e.set_state(0, pos, vel)
+    start, atoms = e.cell_list(0)
+    rstart, ratoms = ljref.cell_list(pos, L, 2.5)
+    assert np.array_equal(start, rstart) and np.array_equal(atoms, ratoms)
+    assert np.diff(start).max() == 8
+    rec = e.step(20)[:, 0, :]
+    start, atoms = e.cell_list(0)
+    p_end = e.get_state(0)["pos"]
+    e.close()
+    rstart, ratoms = ljref.cell_list(p_end, L, 2.5)
+    assert np.array_equal(start, rstart) and np.array_equal(atoms, ratoms)
+    err = np.abs(rec - ref) / np.maximum(np.abs(ref), 1e-300)
+    _say("slab passes", precision, "max rel err", err.max())
+    assert err.max() < (1e-11 if precision == "fp64" else 1e-5)
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_crowded_cells_track_oracle(precision):
     """Cutoff 3.9 sigma on a 500-particle liquid (L = 8.40): a 2^3 cell grid of
     about 62 particles per cell, so cells often receive more arrivals than
