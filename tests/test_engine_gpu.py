@@ -112,10 +112,13 @@ def _oracle_counts(pos, L, rc=2.5):
 
 @pytest.mark.parametrize("kernel", ["tiled", "particle"])
 @pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.3), (500, 1.2, 0.4), (40, 0.05, 0.5),
-                                       (4000, 0.3, 1.0), (256000, 0.8442, 0.2)])
+                                       (4000, 0.3, 1.0), (256000, 0.8442, 0.2),
+                                       (32, 1.185, 0.1)])
 def test_production_kernel_cutoff_decisions_bit_exact(n, rho, amp, kernel):
     """Per-particle in-cutoff neighbour counts of the production pair kernels
-    (FP64) equal the reference filter's, particle by particle."""
+    (FP64) equal the reference filter's, particle by particle (N = 32 at
+    rho 1.185: L = 3.0, one cell per axis, all 27 neighbour cells are images
+    of the home cell; N = 500 at rho 1.2: two per axis)."""
     eng = _engine_mod()
     _say("neighbour counts", n, rho, kernel)
     pos, L = perturbed_lattice(n, rho, amp, 17)
@@ -145,7 +148,7 @@ def test_mixed_kernel_cutoff_decisions(kernel):
 
 @pytest.mark.parametrize("kernel", ["tiled", "particle"])
 @pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.2), (4000, 0.8, 0.3), (500, 1.2, 0.2),
-                                       (256000, 0.8442, 0.15)])
+                                       (256000, 0.8442, 0.15), (32, 1.185, 0.1)])
 def test_forces_fp64(n, rho, amp, kernel):
     eng = _engine_mod()
     _say("forces", n, kernel)
