@@ -866,26 +866,30 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
         } else {
             // cell-local float coordinates; sep = (l_i - l_j) + shift, where
             // shift = (image * nc - step) * a - image * L for the neighbour cell
+            // The LJ law as in the tiled mixed kernel: F / 24 accumulated from
+            // g / 24 = s^6 (2 s^6 - 1), with per-particle float sums of g, s^6
+            // and r^2 from which V and W follow in double.
             int cx, cy, cz;
             cell_xyz_of(p.cell_xyz, p.scell[s], cx, cy, cz);
             const float4 li = p.lpos[s];
-            const float rc2f = (float)S.rc2;
-            const float inv_s2 = (float)(1.0 / S.sigma2);
-            const float ur = (float)(1.0 / (S.sigma2 * S.rc2_red));
-            const float al = (float)S.alpha, be = (float)S.beta;
+            const float rc2f = S.rc2f;
+            const float inv_s2 = S.inv_s2f;
+            const float cw24 = S.cw24f;
             float ax = 0.f, ay = 0.f, az = 0.f;
+            float gs = 0.f, s6s = 0.f, r2s = 0.f;   // LJ sums
+            float spe = 0.f, sww = 0.f;             // constant law
             for (int ox = -1; ox <= 1; ++ox) {
                 int nbx, ix;
                 nb_axis(cx, ox, S.nc[0], nbx, ix);
-                const float shx = (float)((ix * S.nc[0] - ox) * S.a[0] - ix * S.L[0]);
+                const float shx = S.shf[0][3 * (ox + 1) + ix + 1];
                 for (int oy = -1; oy <= 1; ++oy) {
                     int nby, iy;
                     nb_axis(cy, oy, S.nc[1], nby, iy);
-                    const float shy = (float)((iy * S.nc[1] - oy) * S.a[1] - iy * S.L[1]);
+                    const float shy = S.shf[1][3 * (oy + 1) + iy + 1];
                     for (int oz = -1; oz <= 1; ++oz) {
                         int nbz, iz;
                         nb_axis(cz, oz, S.nc[2], nbz, iz);
-                        const float shz = (float)((iz * S.nc[2] - oz) * S.a[2] - iz * S.L[2]);
+                        const float shz = S.shf[2][3 * (oz + 1) + iz + 1];
                         const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
                         const int j0 = p.start[nb], j1 = p.start[nb + 1];
                         for (int j = j0; j < j1; ++j) {
@@ -901,36 +905,42 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
                                     const float inv = __builtin_amdgcn_rcpf(r2);
                                     const float t = inv * inv_s2;
                                     const float s6 = t * t * t;
-                                    const float u = r2 * ur;
-                                    const float vv = __builtin_fmaf(4.0f * s6, s6 - 1.0f,
-                                                                    __builtin_fmaf(be, u - 1.0f, al));
-                                    const float ww = __builtin_fmaf(24.0f * s6, 2.0f * s6 - 1.0f,
-                                                                    -2.0f * be * u);
-                                    const float fm = ww * inv;
+                                    const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
+                                    gs += g;
+                                    s6s += s6;
+                                    r2s += r2;
+                                    const float fm = __builtin_fmaf(g, inv, cw24);
                                     ax = __builtin_fmaf(fm, dx, ax);
                                     ay = __builtin_fmaf(fm, dy, ay);
                                     az = __builtin_fmaf(fm, dz, az);
-                                    pe += (double)vv;
-                                    w += (double)ww;
                                 } else {
                                     const float nrm = sqrtf(r2);
                                     const float vv = (float)S.strength * ((float)S.rc - nrm);
                                     const float ww = (float)S.strength * nrm;
                                     const float fm = ww / r2;
                                     ax += fm * dx; ay += fm * dy; az += fm * dz;
-                                    pe += (double)vv;
-                                    w += (double)ww;
+                                    spe += vv;
+                                    sww += ww;
                                 }
                             }
                         }
                     }
                 }
             }
-            if (LAW == LAW_LJ && S.eps != 1.0) {
-                ax *= (float)S.eps; ay *= (float)S.eps; az *= (float)S.eps;
-                pe *= S.eps; w *= S.eps;
+            if (LAW == LAW_LJ) {
+                // V = 4 s^12 - 4 s^6 + (alpha - beta) + beta u = 2 g' - 2 s^6 + ...,
+                // W = 24 g' - 2 beta u (g' = g / 24), F = 24 (F / 24), times epsilon
+                const double u_sum = (double)r2s * S.inv_rc2s;
+                pe = S.eps * (2.0 * (double)gs - 2.0 * (double)s6s +
+                              (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
+                w = S.eps * (24.0 * (double)gs - 2.0 * S.beta * u_sum);
+                const double fs = 24.0 * S.eps;
+                fx = fs * (double)ax; fy = fs * (double)ay; fz = fs * (double)az;
+            } else {
+                pe = spe;
+                w = sww;
+                fx = ax; fy = ay; fz = az;
             }
-            fx = ax; fy = ay; fz = az;
         }
     }
     double acc[4] = {0, 0, 0, 0};
