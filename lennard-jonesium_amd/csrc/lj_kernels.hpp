@@ -878,6 +878,38 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
             float ax = 0.f, ay = 0.f, az = 0.f;
             float gs = 0.f, s6s = 0.f, r2s = 0.f;   // LJ sums
             float spe = 0.f, sww = 0.f;             // constant law
+            auto visit = [&](int j, const float4 lj, float shx, float shy, float shz) {
+                if (j == s) return;
+                const float dx = (li.x - lj.x) + shx;
+                const float dy = (li.y - lj.y) + shy;
+                const float dz = (li.z - lj.z) + shz;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                if (r2 < rc2f) {
+                    ++nct;
+                    if (LAW == LAW_LJ) {
+                        const float inv = __builtin_amdgcn_rcpf(r2);
+                        const float t = inv * inv_s2;
+                        const float s6 = t * t * t;
+                        const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
+                        gs += g;
+                        s6s += s6;
+                        r2s += r2;
+                        const float fm = __builtin_fmaf(g, inv, cw24);
+                        ax = __builtin_fmaf(fm, dx, ax);
+                        ay = __builtin_fmaf(fm, dy, ay);
+                        az = __builtin_fmaf(fm, dz, az);
+                    } else {
+                        const float nrm = sqrtf(r2);
+                        const float vv = (float)S.strength * ((float)S.rc - nrm);
+                        const float ww = (float)S.strength * nrm;
+                        const float fm = ww / r2;
+                        ax += fm * dx; ay += fm * dy; az += fm * dz;
+                        spe += vv;
+                        sww += ww;
+                    }
+                }
+            };
+            const int last = p.nslot - 1;
             for (int ox = -1; ox <= 1; ++ox) {
                 int nbx, ix;
                 nb_axis(cx, ox, S.nc[0], nbx, ix);
@@ -886,44 +918,27 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
                     int nby, iy;
                     nb_axis(cy, oy, S.nc[1], nby, iy);
                     const float shy = S.shf[1][3 * (oy + 1) + iy + 1];
-                    for (int oz = -1; oz <= 1; ++oz) {
+                    // the row's three cells: their ranges, then their first
+                    // members, each fetched in one round (dilute systems hold
+                    // a particle or two per cell)
+                    int jb[3], je[3];
+                    float shz[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
                         int nbz, iz;
-                        nb_axis(cz, oz, S.nc[2], nbz, iz);
-                        const float shz = S.shf[2][3 * (oz + 1) + iz + 1];
+                        nb_axis(cz, k - 1, S.nc[2], nbz, iz);
+                        shz[k] = S.shf[2][3 * k + iz + 1];
                         const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
-                        const int j0 = p.start[nb], j1 = p.start[nb + 1];
-                        for (int j = j0; j < j1; ++j) {
-                            if (j == s) continue;
-                            const float4 lj = p.lpos[j];
-                            const float dx = (li.x - lj.x) + shx;
-                            const float dy = (li.y - lj.y) + shy;
-                            const float dz = (li.z - lj.z) + shz;
-                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                            if (r2 < rc2f) {
-                                ++nct;
-                                if (LAW == LAW_LJ) {
-                                    const float inv = __builtin_amdgcn_rcpf(r2);
-                                    const float t = inv * inv_s2;
-                                    const float s6 = t * t * t;
-                                    const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
-                                    gs += g;
-                                    s6s += s6;
-                                    r2s += r2;
-                                    const float fm = __builtin_fmaf(g, inv, cw24);
-                                    ax = __builtin_fmaf(fm, dx, ax);
-                                    ay = __builtin_fmaf(fm, dy, ay);
-                                    az = __builtin_fmaf(fm, dz, az);
-                                } else {
-                                    const float nrm = sqrtf(r2);
-                                    const float vv = (float)S.strength * ((float)S.rc - nrm);
-                                    const float ww = (float)S.strength * nrm;
-                                    const float fm = ww / r2;
-                                    ax += fm * dx; ay += fm * dy; az += fm * dz;
-                                    spe += vv;
-                                    sww += ww;
-                                }
-                            }
-                        }
+                        jb[k] = p.start[nb];
+                        je[k] = p.start[nb + 1];
+                    }
+                    float4 l0[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) l0[k] = p.lpos[min(jb[k], last)];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (jb[k] < je[k]) visit(jb[k], l0[k], shx, shy, shz[k]);
+                        for (int j = jb[k] + 1; j < je[k]; ++j) visit(j, p.lpos[j], shx, shy, shz[k]);
                     }
                 }
             }
