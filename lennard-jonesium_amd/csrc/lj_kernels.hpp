@@ -1559,7 +1559,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
             while (g0 < total) {
                 // ---- stage the surviving neighbours of the next candidates, 64 at a time
                 int m = 0;
-                while (g0 < total && m <= cap - 64) {
+                // A stage is filled to exactly cap entries: of the batch that
+                // overflows it, the survivors past the room are left for the
+                // next stage, which restarts after the last one taken.
+                while (g0 < total && m < cap) {
                     const int cq = nq;
                     const double cxd = nx, cyd = ny, czd = nz;
                     const int g = g0 + lane;
@@ -1576,8 +1579,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                         g < total && __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) < keepf;
                     rv.w = __builtin_fmaf(rv.z, rv.z, __builtin_fmaf(rv.y, rv.y, rv.x * rv.x));
                     const unsigned long long bal = __ballot(keep);
-                    if (keep) {
-                        const unsigned t = (unsigned)m + (unsigned)__popcll(bal & below);
+                    const int rank = __popcll(bal & below);
+                    const int room = cap - m;   // >= 1
+                    if (keep && rank < room) {
+                        const unsigned t = (unsigned)(m + rank);
                         const unsigned row = (t * magic) >> 20;
                         const unsigned e = (t - row * (unsigned)nsl) * (unsigned)SL + row;
                         sw[e] = rv;
@@ -1586,7 +1591,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                         sdw[3 * e + 2] = czd;
                         sq[wid][e] = (unsigned char)cq;
                     }
-                    m += __popcll(bal);
+                    const int nk = __popcll(bal);
+                    if (nk > room) {
+                        const unsigned long long lastm = __ballot(keep && rank == room - 1);
+                        g0 += __builtin_ctzll(lastm) + 1 - 64;   // after the last one taken
+                        fetch(g0 + lane, nq, nx, ny, nz);
+                        m = cap;
+                    } else {
+                        m += nk;
+                    }
                 }
                 const int kmax = __builtin_amdgcn_readfirstlane(
                     (int)(((unsigned)(m + nsl - 1) * magic) >> 20));
