@@ -121,6 +121,10 @@ struct lj_engine {
     int* d_ncount = nullptr;    // set only while a neighbour-count hook runs
     int cur = 0;                // which buffer set holds the current state
     bool sorted = false;        // slots are in cell order (after a cell build)
+    // Inside a chained step sequence: the stored velocities already carry the
+    // next step's first half kick and the stored forces are stale.  False
+    // between API calls.
+    bool prekicked = false;
     hipStream_t stream = nullptr;
     bool use_graphs = true;
     hipGraphExec_t graph2 = nullptr;   // two steps starting from buffer set 0
@@ -279,12 +283,13 @@ void launch(lj_engine* e, void (*k)(Params...), dim3 grid, dim3 block, Args... a
 
 int enqueue_finalize(lj_engine* e, int mode);
 
-// Cell build, optionally for the integrated positions (first half kick,
-// drift, wrap): the key pass evaluates the integrated position for the keys
+// Cell build, optionally for the integrated positions (integrate 1: first
+// half kick, drift, wrap; 2: drift and wrap of velocities that already carry
+// the half kick): the key pass evaluates the integrated position for the keys
 // only (and records each cell's arrivals), and the sort's move into the other
 // buffer set applies the same integration.  Between them: the scan.
 // Leaves e->cur pointing at the sorted set.
-int enqueue_cells(lj_engine* e, bool integrate)
+int enqueue_cells(lj_engine* e, int integrate)
 {
     const ljk::StepArgs p = e->args(e->cur);
     const dim3 gb(e->nblk), tb(kBlock);
@@ -294,7 +299,11 @@ int enqueue_cells(lj_engine* e, bool integrate)
     ljk::FinalArgs f0 = e->fargs(0);
     const int nfin = integrate ? e->nsys * f0.nf : 0;
     int rc = LJ_OK;
-    if (integrate)
+    if (integrate == 2)
+        rc = timed(e, 0, "integrate_key", [&] {
+            launch(e, ljk::k_integrate_key<3, true>, dim3(e->nblk + nfin), tb, p, f0, nfin);
+        });
+    else if (integrate)
         rc = timed(e, 0, "integrate_key", [&] {
             launch(e, ljk::k_integrate_key<2, true>, dim3(e->nblk + nfin), tb, p, f0, nfin);
         });
@@ -323,10 +332,15 @@ int enqueue_cells(lj_engine* e, bool integrate)
     const bool mixed = e->precision == LJ_PRECISION_MIXED;
     const dim3 gc(e->nsblk);
     rc = timed(e, 4, "sort", [&] {
-        if (mixed && integrate) launch(e, ljk::k_sort_cells<true, true>, gc, tb, p);
-        else if (mixed) launch(e, ljk::k_sort_cells<true, false>, gc, tb, p);
-        else if (integrate) launch(e, ljk::k_sort_cells<false, true>, gc, tb, p);
-        else launch(e, ljk::k_sort_cells<false, false>, gc, tb, p);
+        if (mixed) {
+            if (integrate == 2) launch(e, ljk::k_sort_cells<true, 2>, gc, tb, p);
+            else if (integrate) launch(e, ljk::k_sort_cells<true, 1>, gc, tb, p);
+            else launch(e, ljk::k_sort_cells<true, 0>, gc, tb, p);
+        } else {
+            if (integrate == 2) launch(e, ljk::k_sort_cells<false, 2>, gc, tb, p);
+            else if (integrate) launch(e, ljk::k_sort_cells<false, 1>, gc, tb, p);
+            else launch(e, ljk::k_sort_cells<false, 0>, gc, tb, p);
+        }
     });
     if (rc) return rc;
     e->cur ^= 1;
@@ -334,14 +348,14 @@ int enqueue_cells(lj_engine* e, bool integrate)
     return LJ_OK;
 }
 
-template <int PREC, int LAW, bool KICK>
+template <int PREC, int LAW, int KICK>
 void launch_pair(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = p.blk_list ? e->n_blk_list : e->nblk;
     launch(e, ljk::k_pair<PREC, LAW, KICK>, dim3(grid), dim3(kBlock), p);
 }
 
-template <int PREC, int LAW, bool KICK>
+template <int PREC, int LAW, int KICK>
 void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = p.cblk_list ? e->n_cblk_list : e->ncblk;
@@ -351,14 +365,14 @@ void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
         launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(kBlock), p);
 }
 
-template <int LAW, bool KICK>
+template <int LAW, int KICK>
 void launch_tiled(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 {
     if (mixed) launch_pair_cells<ljk::MIXED, LAW, KICK>(e, p);
     else launch_pair_cells<ljk::FP64, LAW, KICK>(e, p);
 }
 
-template <int LAW, bool KICK>
+template <int LAW, int KICK>
 void launch_particle(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 {
     if (mixed) launch_pair<ljk::MIXED, LAW, KICK>(e, p);
@@ -367,7 +381,9 @@ void launch_particle(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 
 // Pair forces (or, for LJ_LAW_NONE engines / measure, just the sums): the
 // tiled kernel for systems flagged tiled, the per-particle kernel for the rest.
-int enqueue_pair(lj_engine* e, bool kick, bool measure_only)
+// kick 1: second half kick; 2: that kick and the next step's first half kick
+// (a chained step; forces are not stored).
+int enqueue_pair(lj_engine* e, int kick, bool measure_only)
 {
     ljk::StepArgs p = e->args(e->cur);
     const int law = measure_only ? ljk::LAW_NONE : e->sys[0].law;
@@ -375,25 +391,29 @@ int enqueue_pair(lj_engine* e, bool kick, bool measure_only)
     e->last_pair_force = law != ljk::LAW_NONE;
     if (law == ljk::LAW_NONE)
         return timed(e, 5, "pair", [&] {
-            if (kick) launch_pair<ljk::FP64, ljk::LAW_NONE, true>(e, p);
-            else launch_pair<ljk::FP64, ljk::LAW_NONE, false>(e, p);
+            if (kick) launch_pair<ljk::FP64, ljk::LAW_NONE, 1>(e, p);
+            else launch_pair<ljk::FP64, ljk::LAW_NONE, 0>(e, p);
         });
     auto tiled = [&] {
         if (law == ljk::LAW_LJ) {
-            if (kick) launch_tiled<ljk::LAW_LJ, true>(e, p, mixed);
-            else launch_tiled<ljk::LAW_LJ, false>(e, p, mixed);
+            if (kick == 2) launch_tiled<ljk::LAW_LJ, 2>(e, p, mixed);
+            else if (kick) launch_tiled<ljk::LAW_LJ, 1>(e, p, mixed);
+            else launch_tiled<ljk::LAW_LJ, 0>(e, p, mixed);
         } else {
-            if (kick) launch_tiled<ljk::LAW_CONSTANT, true>(e, p, mixed);
-            else launch_tiled<ljk::LAW_CONSTANT, false>(e, p, mixed);
+            if (kick == 2) launch_tiled<ljk::LAW_CONSTANT, 2>(e, p, mixed);
+            else if (kick) launch_tiled<ljk::LAW_CONSTANT, 1>(e, p, mixed);
+            else launch_tiled<ljk::LAW_CONSTANT, 0>(e, p, mixed);
         }
     };
     auto particle = [&] {
         if (law == ljk::LAW_LJ) {
-            if (kick) launch_particle<ljk::LAW_LJ, true>(e, p, mixed);
-            else launch_particle<ljk::LAW_LJ, false>(e, p, mixed);
+            if (kick == 2) launch_particle<ljk::LAW_LJ, 2>(e, p, mixed);
+            else if (kick) launch_particle<ljk::LAW_LJ, 1>(e, p, mixed);
+            else launch_particle<ljk::LAW_LJ, 0>(e, p, mixed);
         } else {
-            if (kick) launch_particle<ljk::LAW_CONSTANT, true>(e, p, mixed);
-            else launch_particle<ljk::LAW_CONSTANT, false>(e, p, mixed);
+            if (kick == 2) launch_particle<ljk::LAW_CONSTANT, 2>(e, p, mixed);
+            else if (kick) launch_particle<ljk::LAW_CONSTANT, 1>(e, p, mixed);
+            else launch_particle<ljk::LAW_CONSTANT, 0>(e, p, mixed);
         }
     };
     // with both kernels in use, each launches only over its own systems' blocks
@@ -427,11 +447,17 @@ int enqueue_finalize(lj_engine* e, int mode)
 // cell build (or before the next step's integration without cells) or by
 // enqueue_finalize(e, 0), which must follow the last step of a sequence
 // before anything else reads records, time or the block sums.
-int enqueue_step(lj_engine* e)
+// chain: another step follows in the same launch sequence with nothing in
+// between, so this step's pair kernel may apply that step's first half kick
+// (forces are then not stored).  The last step of a sequence leaves the
+// state as the reference has it: full-step velocities and forces.
+int enqueue_step(lj_engine* e, bool chain = false)
 {
     int rc;
     if (e->has_cells) {
-        if ((rc = enqueue_cells(e, true))) return rc;
+        if ((rc = enqueue_cells(e, e->prekicked ? 2 : 1))) return rc;
+        e->prekicked = chain;
+        return enqueue_pair(e, chain ? 2 : 1, false);
     } else {
         const ljk::StepArgs p = e->args(e->cur);
         if ((rc = enqueue_finalize(e, 0))) return rc;
@@ -468,13 +494,15 @@ int capture_steps(lj_engine* e, int nsteps, hipGraphExec_t* out)
     const int saved = e->cur;
     const bool saved_sorted = e->sorted;
     e->cur = 0;
+    e->prekicked = false;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = LJ_OK;
-    for (int s = 0; s < nsteps && !rc; ++s) rc = enqueue_step(e);
+    for (int s = 0; s < nsteps && !rc; ++s) rc = enqueue_step(e, s + 1 < nsteps);
     hipError_t ce = hipStreamEndCapture(e->stream, &g);
     e->cur = saved;
     e->sorted = saved_sorted;
+    e->prekicked = false;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -930,7 +958,7 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
         }
     }
     while (done < nsteps) {
-        if ((rc = enqueue_step(e))) return rc;
+        if ((rc = enqueue_step(e, done + 1 < nsteps))) return rc;
         ++done;
     }
     if ((rc = enqueue_finalize(e, 0))) return rc;   // the last step's record

@@ -258,15 +258,20 @@ __device__ void finalize_part(const FinalArgs& f, int k, int r);
 // engine/integrator.cpp:69-78, engine/boundary_condition.cpp:45-48):
 // v += (1/2) F dt with the stored force (zero before the first calculation),
 // then r += v dt and d += v dt (the same increment).  The cell-key pass and
-// the sort evaluate it identically.
+// the sort evaluate it identically.  KICK1 false: the velocities already
+// carry the half kick (a chained step, whose previous pair kernel applied it
+// with this arithmetic and the same force), and fx, fy, fz are unused.
+template <bool KICK1 = true>
 __device__ __forceinline__ void advance(const SysDev& S, double& x, double& y, double& z,
                                         double& vx, double& vy, double& vz, double fx, double fy,
                                         double fz, double& ix, double& iy, double& iz)
 {
     const double dt = S.dt;
-    vx = vx + ((1. / 2.) * fx) * dt;
-    vy = vy + ((1. / 2.) * fy) * dt;
-    vz = vz + ((1. / 2.) * fz) * dt;
+    if (KICK1) {
+        vx = vx + ((1. / 2.) * fx) * dt;
+        vy = vy + ((1. / 2.) * fy) * dt;
+        vz = vz + ((1. / 2.) * fz) * dt;
+    }
     ix = vx * dt; iy = vy * dt; iz = vz * dt;
     x = x + ix; y = y + iy; z = z + iz;
     if (S.periodic) {
@@ -291,7 +296,8 @@ __device__ __forceinline__ void record_member(const StepArgs& p, int s, int c, i
 
 // MODE 0: keys of the current positions; 1: integrate in place (and key);
 // 2: key of the integrated position only -- the sort applies the step while
-// it moves the particle, so the state is read once and written once.
+// it moves the particle, so the state is read once and written once; 3: as
+// 2 for velocities that already carry the half kick (no force is read).
 // The first nfin blocks compute the first level of the previous step's
 // record sums instead (f.nsys * f.nf parts, section 7); the scan's extra
 // blocks finish them.
@@ -311,8 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_key(StepArgs p, FinalArgs 
         double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
         if (MODE != 0 && S.active) {
             double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+            double fx = 0., fy = 0., fz = 0.;
+            if (MODE != 3) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
             double ix, iy, iz;
-            advance(S, x, y, z, vx, vy, vz, p.force.x[s], p.force.y[s], p.force.z[s], ix, iy, iz);
+            advance<MODE != 3>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
             if (MODE == 1) {
                 p.disp.x[s] = p.disp.x[s] + ix;
                 p.disp.y[s] = p.disp.y[s] + iy;
@@ -566,7 +574,7 @@ __global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalA
 //    writes cell-local float coordinates.  The sorted order is a function of
 //    the positions alone, whatever order the key pass's atomics took.
 // ---------------------------------------------------------------------------
-template <bool MIXED_, bool INTEGRATE>
+template <bool MIXED_, int INTEGRATE>
 __device__ __forceinline__ void move_particle(const StepArgs& p, const SysDev& S, int c, int cx,
                                               int cy, int cz, int s, int me, int dst)
 {
@@ -574,8 +582,10 @@ __device__ __forceinline__ void move_particle(const StepArgs& p, const SysDev& S
     double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
     double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
     if (INTEGRATE && S.active) {
+        double fx = 0., fy = 0., fz = 0.;
+        if (INTEGRATE == 1) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
         double ix, iy, iz;
-        advance(S, x, y, z, vx, vy, vz, p.force.x[s], p.force.y[s], p.force.z[s], ix, iy, iz);
+        advance<INTEGRATE == 1>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
         dx = dx + ix; dy = dy + iy; dz = dz + iz;
     }
     float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -593,7 +603,7 @@ __device__ __forceinline__ void move_particle(const StepArgs& p, const SysDev& S
 // One cell (local index lc) of n members from slot c0, one wave: the path for
 // cells that do not fit a shared wave.  Lane j holds the j-th arrival; cells
 // with more members than records take the rest from the overflow list.
-template <bool MIXED_, bool INTEGRATE>
+template <bool MIXED_, int INTEGRATE>
 __device__ __forceinline__ void sort_one_cell(const StepArgs& p, const SysDev& S, int lc, int c0,
                                               int n)
 {
@@ -654,7 +664,7 @@ __device__ __forceinline__ void sort_one_cell(const StepArgs& p, const SysDev& S
 // one is recorded): lane l holds member l - off(k) of cell k (off: the cell's
 // start relative to cell k0's), and ranks by id are counted among the lanes
 // of the same cell.  st: lane k holds the start of the group's cell k.
-template <bool MIXED_, bool INTEGRATE>
+template <bool MIXED_, int INTEGRATE>
 __device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, int cf, int st,
                                            int k0, int k1, int maxcnt)
 {
@@ -677,7 +687,7 @@ __device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, i
     double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
     double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
     double fx = 0., fy = 0., fz = 0.;
-    if (INTEGRATE) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
+    if (INTEGRATE == 1) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
     int cx = 0, cy = 0, cz = 0;
     if (MIXED_) cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
     // rank: members of my cell (lanes jb .. je - 1) with a smaller id
@@ -692,7 +702,7 @@ __device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, i
         const int dst = g0 + jb + rank;
         if (INTEGRATE && S.active) {
             double ix, iy, iz;
-            advance(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+            advance<INTEGRATE == 1>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
             dx = dx + ix; dy = dy + iy; dz = dz + iz;
         }
         float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -712,7 +722,7 @@ __device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, i
 // members fit 64 lanes are sorted together (sort_group; normally the whole
 // group in one pass); when a cell has more members than records, the wave
 // walks the cells one by one (sort_one_cell).
-template <bool MIXED_, bool INTEGRATE>
+template <bool MIXED_, int INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 {
     const int b = blockIdx.x;
@@ -844,7 +854,30 @@ __device__ __forceinline__ int xcd_slab_block()
     return x * q + min(x, r) + b / kXcds;
 }
 
-template <int PREC, int LAW, bool KICK>
+// Second half kick of slot s (engine/integrator.cpp:80-81) with the force just
+// computed: v += (1/2) F dt, left in vx, vy, vz for the kinetic energy.  For a
+// chained step (CHAIN: another step follows in the same launch sequence) the
+// stored velocity also takes the next step's first half kick with the same
+// force and the same arithmetic, v + (1/2) F dt, so that step's key pass and
+// sort read no force; the force is then not stored at all.
+__device__ __forceinline__ void kick_store(const StepArgs& p, const SysDev& S, int s, double fx,
+                                           double fy, double fz, double& vx, double& vy,
+                                           double& vz, bool chain)
+{
+    const double dt = S.dt;
+    const double hx = ((1. / 2.) * fx) * dt, hy = ((1. / 2.) * fy) * dt,
+                 hz = ((1. / 2.) * fz) * dt;
+    vx = vx + hx;
+    vy = vy + hy;
+    vz = vz + hz;
+    if (chain) {
+        p.vel.x[s] = vx + hx; p.vel.y[s] = vy + hy; p.vel.z[s] = vz + hz;
+    } else {
+        p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+    }
+}
+
+template <int PREC, int LAW, int KICK>
 __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 {
     const int b = p.blk_list ? p.blk_list[xcd_slab_block()] : xcd_slab_block();
@@ -961,15 +994,11 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
     double acc[4] = {0, 0, 0, 0};
     if (active) {
         if (p.ncount) p.ncount[s] = nct;
-        if (LAW != LAW_NONE) { p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz; }
-        double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
-        if (KICK) {
-            const double dt = S.dt;
-            vx = vx + ((1. / 2.) * fx) * dt;
-            vy = vy + ((1. / 2.) * fy) * dt;
-            vz = vz + ((1. / 2.) * fz) * dt;
-            p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
+        if (LAW != LAW_NONE && KICK != 2) {
+            p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz;
         }
+        double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+        if (KICK) kick_store(p, S, s, fx, fy, fz, vx, vy, vz, KICK == 2);
         const double ddx = p.disp.x[s], ddy = p.disp.y[s], ddz = p.disp.z[s];
         acc[0] = (vx * vx + vy * vy) + vz * vz;
         acc[1] = (1. / 2.) * pe;
@@ -1084,7 +1113,7 @@ struct SliceGeomTable {
 __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 
 
-template <int LAW, bool KICK>
+template <int LAW, int KICK>
 __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
 {
     constexpr int NW = kBlock / 64;
@@ -1362,15 +1391,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
                 const double fx = fscale * (double)ax, fy = fscale * (double)ay,
                              fz = fscale * (double)az;
                 if (p.ncount) p.ncount[si] = nct;
-                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
+                if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-                if (KICK) {
-                    const double dt = S.dt;
-                    vx = vx + ((1. / 2.) * fx) * dt;
-                    vy = vy + ((1. / 2.) * fy) * dt;
-                    vz = vz + ((1. / 2.) * fz) * dt;
-                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
-                }
+                if (KICK) kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
                 const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
                 wacc[0] += (vx * vx + vy * vy) + vz * vz;
                 wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
@@ -1427,7 +1450,7 @@ struct SliceGeomTableD {
 
 __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 
-template <int LAW, bool KICK>
+template <int LAW, int KICK>
 __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
 {
     constexpr int NW = kBlock / 64;
@@ -1698,15 +1721,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                     nct += (int)sdw[o + 3];
                 }
                 if (p.ncount) p.ncount[si] = nct;
-                p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz;
+                if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-                if (KICK) {
-                    const double dt = S.dt;
-                    vx = vx + ((1. / 2.) * fx) * dt;
-                    vy = vy + ((1. / 2.) * fy) * dt;
-                    vz = vz + ((1. / 2.) * fz) * dt;
-                    p.vel.x[si] = vx; p.vel.y[si] = vy; p.vel.z[si] = vz;
-                }
+                if (KICK) kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
                 const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
                 wacc[0] += (vx * vx + vy * vy) + vz * vz;
                 wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;

@@ -461,6 +461,36 @@ def test_records_independent_of_call_boundaries(graphs):
     assert np.all(np.diff(runs[0][:, :, 0], axis=0) > 0)   # time advances every step
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_chained_steps_leave_the_reference_state(precision, kernel):
+    """Inside one call the pair kernel also applies the next step's first half
+    kick (forces are then not stored); the last step of a call must leave
+    full-step velocities and the forces, bit for bit as one step per call
+    does, which applies each half kick from the stored force.  Eager
+    launches, so that the chain spans all 37 steps."""
+    eng = _engine_mod()
+    st, L = ljref.initial_condition(864, 0.8, 1.0, seed=11)
+    out = []
+    for split in ((37,), (1,) * 37):
+        e = eng.Engine([eng.SystemSpec(864, L)], precision=precision, graphs=False,
+                       pair_kernel=kernel)
+        e.set_state(0, st.pos, st.vel)
+        rec = np.concatenate([e.step(k) for k in split], axis=0)
+        out.append((rec, e.get_state(0)))
+        e.close()
+    (r0, s0), (r1, s1) = out
+    assert np.array_equal(r0, r1)
+    for key in ("pos", "vel", "disp", "force"):
+        assert np.array_equal(s0[key], s1[key]), key
+    assert s0["time"] == s1["time"] and s0["pe"] == s1["pe"] and s0["vir"] == s1["vir"]
+    # and the chained run still tracks the oracle's first steps
+    if precision == "fp64":
+        ref = ljref.run(st.copy(), L, 0.005, 37)
+        rel = np.abs(r0[:, 0, :] - ref) / np.abs(ref)
+        assert rel.max() < 1e-11
+
+
 def test_large_cell_grid_records_track_oracle():
     """More than 32768 cells (dilute box, 37^3 cells): the two-launch cell scan
     and the two-level record sums; FP64 records follow the oracle, and split
