@@ -491,6 +491,39 @@ def test_chained_steps_leave_the_reference_state(precision, kernel):
         assert rel.max() < 1e-11
 
 
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_inactive_replica_inside_chained_graphs(kernel):
+    """Step graphs are captured while every replica is active; a replica
+    switched off later must stay where it is (its particles still enter the
+    next cell builds) while the other one runs on exactly as alone."""
+    eng = _engine_mod()
+    cases = [(500, 0.8, 0.7, 3), (864, 0.8, 0.9, 5)]
+    specs, states = [], []
+    for n, rho, T, seed in cases:
+        st, L = ljref.initial_condition(n, rho, T, seed=seed)
+        specs.append(eng.SystemSpec(n, L))
+        states.append(st)
+    e = eng.Engine(specs, precision="mixed", pair_kernel=kernel)
+    for k, st in enumerate(states):
+        e.set_state(k, st.pos, st.vel)
+    e.step(64)                       # graphs built and replayed, both active
+    held = e.get_state(1)
+    e.set_active(1, False)
+    rec = e.step(70)[:, 0, :]        # 32-step and 2-step graph replays
+    got0, got1 = e.get_state(0), e.get_state(1)
+    e.close()
+    alone = eng.Engine([specs[0]], precision="mixed", pair_kernel=kernel)
+    alone.set_state(0, states[0].pos, states[0].vel)
+    alone.step(64)
+    ref = alone.step(70)[:, 0, :]
+    ref0 = alone.get_state(0)
+    alone.close()
+    assert np.array_equal(rec, ref)
+    for key in ("pos", "vel", "disp", "force"):
+        assert np.array_equal(got0[key], ref0[key]), key
+        assert np.array_equal(got1[key], held[key]), key
+
+
 def test_large_cell_grid_records_track_oracle():
     """More than 32768 cells (dilute box, 37^3 cells): the two-launch cell scan
     and the two-level record sums; FP64 records follow the oracle, and split
