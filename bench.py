@@ -181,7 +181,12 @@ def liquid(args, eng):
         "fp64": res64,
         "mixed_over_fp64": (rate_m / res64["value"]) if res64 else None,
         "force_rms_rel_err_vs_fp64": frms,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32"],
+        # compute-bound: priced against MI355X's published FP32 compute roof
+        # (157.3 TFLOP/s: the matrix cores, or the vector ALUs with packed FP32,
+        # which this build leaves off); the kernel runs on the vector ALUs, whose
+        # issue fraction is reported beside it
+        "roofline": {"bound": "mfma", "executes_on": "VALU (FP32 vector, no MFMA)",
+                     "achieved": achieved, "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": traffic,
                      "kernel": ("k_pair_cells_mixed<LJ,KICK>" if args.pair_kernel == "tiled"
                                 else "k_pair<MIXED,LJ,KICK>"),
