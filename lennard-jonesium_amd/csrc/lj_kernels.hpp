@@ -1488,6 +1488,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
         // ---- the 27 neighbour cells (reference lattice-image rule), one lane each
         int cntq = 0, j0 = 0;
         double sx = 0.0, sy = 0.0, sz = 0.0;
+        bool img = false;
         if (lane < 27) {
             const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
             int nbx, ix, nby, iy, nbz, iz;
@@ -1503,7 +1504,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
             tb_shd[wid][lane][0] = sx;
             tb_shd[wid][lane][1] = sy;
             tb_shd[wid][lane][2] = sz;
+            img = (ix | iy | iz) != 0;
         }
+        // no neighbour cell is an image (cells away from the box faces): every
+        // shift is +0.0, and (r_i - r_j) - 0.0 is r_i - r_j bit for bit
+        const bool noimg = __builtin_amdgcn_ballot_w64(img) == 0ull;
         const int incl = wave_incl_scan_dpp(cntq);
         if (lane < 27) {
             tb_off[wid][lane] = incl - cntq;
@@ -1663,14 +1668,22 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
 
                 // ---- phase 2: the reference's test and law on every candidate that passed
                 const int etop = sl * SL + 63;
-                while (mask) {
+                auto walk = [&](auto noimg_t) {
+                  constexpr bool NOIMG = decltype(noimg_t)::value;
+                  while (mask) {
                     const int bit = __builtin_ctzll(mask);
                     mask &= mask - 1;
                     const int e = etop - bit;
-                    const int q = sq[wid][e];
-                    const double dx = (xi - sdw[3 * e]) - tb_shd[wid][q][0];
-                    const double dy = (yi - sdw[3 * e + 1]) - tb_shd[wid][q][1];
-                    const double dz = (zi - sdw[3 * e + 2]) - tb_shd[wid][q][2];
+                    // the entry's double position (24-bit multiply for the byte offset)
+                    const double* pj = reinterpret_cast<const double*>(
+                        reinterpret_cast<const char*>(sdw) + __umul24((unsigned)e, 24u));
+                    double dx = xi - pj[0], dy = yi - pj[1], dz = zi - pj[2];
+                    if (!NOIMG) {
+                        const int q = sq[wid][e];
+                        dx = dx - tb_shd[wid][q][0];
+                        dy = dy - tb_shd[wid][q][1];
+                        dz = dz - tb_shd[wid][q][2];
+                    }
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (r2 < rc2d && r2 > 0.0) {
                         ++nct;
@@ -1693,7 +1706,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
                             pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
                         }
                     }
-                }
+                  }
+                };
+                if (noimg) walk(std::integral_constant<bool, true>{});
+                else walk(std::integral_constant<bool, false>{});
                 wave_lds_sync();
             }
             if (LAW == LAW_LJ) {
