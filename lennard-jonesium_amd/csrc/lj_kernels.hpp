@@ -721,7 +721,9 @@ __device__ __forceinline__ void sort_group(const StepArgs& p, const SysDev& S, i
 // A wave takes sort_k consecutive cells.  Runs of consecutive cells whose
 // members fit 64 lanes are sorted together (sort_group; normally the whole
 // group in one pass); when a cell has more members than records, the wave
-// walks the cells one by one (sort_one_cell).
+// walks the cells one by one (sort_one_cell).  INTEGRATE 0: move only; 1:
+// first half kick from the stored force, drift and wrap; 2: drift and wrap
+// (a chained step: the velocities already carry the half kick).
 template <bool MIXED_, int INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 {
@@ -767,7 +769,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 
 // ---------------------------------------------------------------------------
 // 6. pair forces (full shell, one thread per sorted particle), fused with the
-//    second half kick and the per-block sums of |v|^2, PE, W, |d|^2.
+//    second half kick and the per-block sums of |v|^2, PE, W, |d|^2.  The
+//    pair kernels' KICK: 0 forces only; 1 second half kick (the force is
+//    stored); 2 also the next step's first half kick (a chained step,
+//    kick_store; the force is not stored).
 //    physics/lennard_jones_force.cpp:76-128, engine/force_calculation.cpp:46-71,
 //    engine/integrator.cpp:81-82, physics/derived_properties.cpp:37-51.
 // ---------------------------------------------------------------------------
