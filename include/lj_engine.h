@@ -117,7 +117,9 @@ int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double*
  * NULL it receives nsteps * nsys lj_thermo rows, step-major
  * (records[step * nsys + sys]), each the measurement of
  * physics/measurements.hpp:59-70 after that step.  The loop runs from a
- * captured HIP graph unless graphs were disabled. */
+ * captured HIP graph unless graphs were disabled.  After a failed call the
+ * systems' states are unspecified until lj_engine_set_state has set each of
+ * them again. */
 int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records);
 
 /* Recomputes forces, potential energy and virial on the current positions

@@ -894,6 +894,9 @@ int lj_engine_set_state(lj_engine* e, int sys, const double* pos, const double* 
     HIP_TRY(hipMemcpy(e->d_pe + sys, &zero, sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_vir + sys, &zero, sizeof(double), hipMemcpyHostToDevice));
     e->sorted = false;
+    // full-step velocities and forces (only a failed step call can leave the
+    // engine inside a chained sequence)
+    e->prekicked = false;
     return LJ_OK;
 }
 
