@@ -360,9 +360,9 @@ void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = p.cblk_list ? e->n_cblk_list : e->ncblk;
     if constexpr (PREC == ljk::MIXED)
-        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(grid), dim3(kBlock), p);
+        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockMixed), p);
     else
-        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(kBlock), p);
+        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockFp64), p);
 }
 
 template <int LAW, int KICK>
@@ -708,7 +708,9 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         s.blk_first = blk;
         s.nblk = slots / kBlock;
         s.cblk_first = cblk;
-        s.ncblk = std::max(1, (s.ncell + ljk::kCellsPerBlock - 1) / ljk::kCellsPerBlock);
+        const int cpb = precision == LJ_PRECISION_MIXED ? ljk::kCellsPerBlockMixed
+                                                        : ljk::kCellsPerBlockFp64;
+        s.ncblk = std::max(1, (s.ncell + cpb - 1) / cpb);
         cblk += s.ncblk;
         // sort: about 44 particles per wave (64 lanes), so that a wave's
         // cells rarely hold more than 64

@@ -30,7 +30,8 @@ namespace ljk {
 
 constexpr int kBlock = 256;        // slots per block in particle-space kernels
 constexpr int kScanTile = 1024;    // cells per block in the scan kernels
-constexpr int kCellsPerBlock = 4;  // one wave64 per home cell in the tiled pair kernel
+constexpr int kCellsPerBlockMixed = 4;  // home cells (one wave64 each) per block, mixed tiled kernel
+constexpr int kCellsPerBlockFp64 = 1;   // FP64 tiled kernel: single-wave blocks retire independently
 constexpr int kHistBins = 1024;    // LDS histogram window of the cell count (cells)
 
 enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
@@ -47,7 +48,7 @@ struct SysDev {
     int periodic;
     int blk_first;     // first particle-space block
     int nblk;          // blocks of this system
-    int cblk_first;    // first cell-space block (kCellsPerBlock cells each)
+    int cblk_first;    // first cell-space block (kCellsPerBlockMixed / kCellsPerBlockFp64 cells each)
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
     int tiled;         // pair forces from the tiled kernel (1) or the per-particle kernel (0)
@@ -1032,10 +1033,10 @@ __device__ __forceinline__ void wave_lds_sync()
 // the four entries 4 (l % 16) .. + 3 of quantity l / 16, a 16-lane DPP
 // reduction completes each quantity within one row, and wave 0 adds the
 // waves' results with a quad DPP reduction.  Thread 0 gets the sums.
+template <int NW = kBlock / 64>
 __device__ __forceinline__ void block_sum4_lds(const double v[4], double* sc, double out[4])
 {
-    constexpr int NW = kBlock / 64;
-    static_assert(NW == 4, "one quad of waves");
+    static_assert(NW == 1 || NW == 2 || NW == 4, "one, two or four waves");
     __shared__ double red[4][NW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     wave_lds_sync();
@@ -1051,18 +1052,23 @@ __device__ __forceinline__ void block_sum4_lds(const double v[4], double* sc, do
     double w4[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) w4[q] = readlane_f64(a, 16 * q);
+    if constexpr (NW == 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = w4[q];
+        return;
+    }
     if (lane == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) red[q][wid] = w4[q];
     }
     __syncthreads();
     if (wid == 0) {
-        // lane 4 q + w holds wave w's quantity q
-        double b = lane < 4 * NW ? red[lane >> 2][lane & 3] : 0.0;
+        // lane NW q + w holds wave w's quantity q
+        double b = lane < 4 * NW ? red[lane / NW][lane % NW] : 0.0;
         b += dpp_f64<0xB1>(b);
-        b += dpp_f64<0x4E>(b);
+        if (NW == 4) b += dpp_f64<0x4E>(b);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) out[q] = readlane_f64(b, 4 * q);
+        for (int q = 0; q < 4; ++q) out[q] = readlane_f64(b, NW * q);
     }
 }
 
@@ -1119,9 +1125,9 @@ __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 
 
 template <int LAW, int KICK>
-__global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
+__global__ __launch_bounds__(64 * kCellsPerBlockMixed, 6) void k_pair_cells_mixed(StepArgs p)
 {
-    constexpr int NW = kBlock / 64;
+    constexpr int NW = kCellsPerBlockMixed;
     // One record per wave, so that every per-lane LDS address is one base
     // register plus an immediate offset.
     struct alignas(16) WaveLds {
@@ -1141,7 +1147,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveLds& W = wl[wid];
-    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
+    const int lc = (b - S.cblk_first) * kCellsPerBlockMixed + wid;
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
     if (lc < S.ncell && (!KICK || S.active)) {
@@ -1410,7 +1416,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_pair_cells_mixed(StepArgs p)
         }
     }
     double out[4];
-    block_sum4_lds(wacc, reinterpret_cast<double*>(W.st), out);
+    block_sum4_lds<NW>(wacc, reinterpret_cast<double*>(W.st), out);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
@@ -1456,9 +1462,9 @@ struct SliceGeomTableD {
 __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 
 template <int LAW, int KICK>
-__global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
+__global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(StepArgs p)
 {
-    constexpr int NW = kBlock / 64;
+    constexpr int NW = kCellsPerBlockFp64;
     __shared__ float4 st[NW][kRowsD];
     __shared__ double sd[NW][kRowsD * 3];      // r_j of each entry; slice and block sums
     __shared__ unsigned char sq[NW][kRowsD];   // table index of the entry's cell
@@ -1473,7 +1479,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
     if (!S.tiled) return;   // this system's forces come from the per-particle kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lc = (b - S.cblk_first) * kCellsPerBlock + wid;
+    const int lc = (b - S.cblk_first) * kCellsPerBlockFp64 + wid;
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
     double* sdw = sd[wid];
 
@@ -1756,7 +1762,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair_cells_fp64(StepArgs p)
         }
     }
     double out[4];
-    block_sum4_lds(wacc, sdw, out);
+    block_sum4_lds<NW>(wacc, sdw, out);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
