@@ -862,7 +862,7 @@ __device__ __forceinline__ int xcd_slab_block()
 
 // Second half kick of slot s (engine/integrator.cpp:80-81) with the force just
 // computed: v += (1/2) F dt, left in vx, vy, vz for the kinetic energy.  For a
-// chained step (CHAIN: another step follows in the same launch sequence) the
+// chained step (chain: another step follows in the same launch sequence) the
 // stored velocity also takes the next step's first half kick with the same
 // force and the same arithmetic, v + (1/2) F dt, so that step's key pass and
 // sort read no force; the force is then not stored at all.
