@@ -291,7 +291,12 @@ def coexistence(args, eng):
     log(f"coexistence: N={n} L={L:.4f} steps={steps}")
     e = eng.Engine([eng.SystemSpec(n, L, cutoff=rc, time_delta=dt)], precision="mixed")
     e.set_state(0, pos, vel)
-    variant, ms = e.autotune(5)
+    # tune on a state the run spends its time in: the quenched lattice breaks up
+    # into liquid and vapour within the first steps, and the fastest kernel on
+    # the lattice is not the fastest afterwards (the timed run still starts
+    # from the lattice)
+    e.step(1000, records=False)
+    variant, ms = e.autotune(20)
     rec, t = timed_run(e, pos, vel, steps, args.warmup)
     rate = n * steps / t
     drift, fluct = drift_stats(rec[:, 0, 3])
