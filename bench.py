@@ -218,7 +218,7 @@ def sweep(args, eng):
     total = sum(m[2] for m in meta)
     log(f"sweep: {len(specs)} replicas, {total} particles, steps={steps}")
     out = {}
-    for prec in ("mixed", "fp64"):
+    for prec in (("mixed",) if args.no_fp64 else ("mixed", "fp64")):
         e = eng.Engine(specs, precision=prec)
         for k, (p, v) in enumerate(states):
             e.set_state(k, p, v)
@@ -259,7 +259,7 @@ def sweep(args, eng):
         e.close()
     # plain per-replica FP64 runs (one engine per replica, as the reference runs one
     # Simulation per thread), timed on a sample of replicas
-    sample = list(range(0, len(specs), 32))
+    sample = list(range(0, len(specs), 32)) if not args.no_plain else []
     t_plain, work = 0.0, 0
     sst = min(steps, 500)
     for k in sample:
@@ -268,18 +268,20 @@ def sweep(args, eng):
         t_plain += t
         work += meta[k][2] * sst
         e.close()
-    plain_rate = work / t_plain
-    log(f"plain per-replica fp64: {plain_rate:.4e} atom-steps/s on {len(sample)} sampled replicas")
+    plain_rate = work / t_plain if sample else None
+    if sample:
+        log(f"plain per-replica fp64: {plain_rate:.4e} atom-steps/s on {len(sample)} sampled replicas")
     return {"metric": METRIC, "value": out["mixed"]["value"], "unit": "atom-steps/s",
             "n_gpus": 1, "steps": steps, "warmup": args.warmup, "higher_is_better": True,
             "dtype": "f32 pair forces + f64 state/sums", "data": "synthetic",
             "config": {"workload": "batched sweep: 256 replicas, 16x16 grid T 0.5-2.0 x rho "
                                    "0.05-1.2, N=4n^3, n in [5,20] from seed 5489, rc 2.5, "
                                    "dt 0.005", "particles": total, "replicas": len(specs)},
-            "mixed_batched": out["mixed"], "fp64_batched": out["fp64"],
+            "mixed_batched": out["mixed"], "fp64_batched": out.get("fp64"),
             "plain_fp64_per_replica": {"value": plain_rate, "sampled_replicas": sample,
                                        "steps": sst},
-            "batched_mixed_over_plain_fp64": out["mixed"]["value"] / plain_rate,
+            "batched_mixed_over_plain_fp64": (out["mixed"]["value"] / plain_rate
+                                              if plain_rate else None),
             "observables": out.get("observables_sample")}
 
 
@@ -390,6 +392,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="sweep: skip the plain per-replica FP64 leg")
     ap.add_argument("--pair-kernel", default="tiled", choices=["tiled", "particle"])
     ap.add_argument("--workload", default="liquid",
                     choices=["liquid", "sweep", "coexistence", "plain", "adjoint"])

@@ -1953,30 +1953,37 @@ __global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bin
         int R[3];
         for (int a = 0; a < 3; ++a) R[a] = (int)ceil(rmax / S.a[a]);
         const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
-        const double rmax2 = rmax * rmax, dr = rmax / bins;
+        const double rmax2 = rmax * rmax, inv_dr = (double)bins / rmax;
+        // Half shell: every stencil offset is a distinct (cell, image), and the
+        // pair (j, i) at the mirrored offset has exactly the negated separation
+        // (IEEE subtraction is sign-symmetric), so it falls in the same bin.
+        // Each unordered pair is visited once -- positive offsets, and j > i in
+        // the home cell -- and counted twice.
         for (int ox = -R[0]; ox <= R[0]; ++ox) {
             int tx = c[0] + ox;
             int ix = (tx >= 0) ? tx / S.nc[0] : -((-tx + S.nc[0] - 1) / S.nc[0]);
             int nbx = tx - ix * S.nc[0];
             for (int oy = -R[1]; oy <= R[1]; ++oy) {
+                if (ox < 0 || (ox == 0 && oy < 0)) continue;
                 int ty = c[1] + oy;
                 int iy = (ty >= 0) ? ty / S.nc[1] : -((-ty + S.nc[1] - 1) / S.nc[1]);
                 int nby = ty - iy * S.nc[1];
                 for (int oz = -R[2]; oz <= R[2]; ++oz) {
+                    const bool home = ox == 0 && oy == 0 && oz == 0;
+                    if (ox == 0 && oy == 0 && oz < 0) continue;
                     int tz = c[2] + oz;
                     int iz = (tz >= 0) ? tz / S.nc[2] : -((-tz + S.nc[2] - 1) / S.nc[2]);
                     int nbz = tz - iz * S.nc[2];
                     const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
                     const double shx = ix * S.L[0], shy = iy * S.L[1], shz = iz * S.L[2];
-                    for (int j = p.start[nb]; j < p.start[nb + 1]; ++j) {
-                        if (j == s && ix == 0 && iy == 0 && iz == 0) continue;
+                    for (int j = home ? s + 1 : p.start[nb]; j < p.start[nb + 1]; ++j) {
                         const double dx = (xi - p.pos.x[j]) - shx;
                         const double dy = (yi - p.pos.y[j]) - shy;
                         const double dz = (zi - p.pos.z[j]) - shz;
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
                         if (r2 < rmax2) {
-                            int bin = (int)(sqrt(r2) / dr);
-                            if (bin < bins) atomicAdd(&lh[bin], 1u);
+                            int bin = (int)(sqrt(r2) * inv_dr);
+                            if (bin < bins) atomicAdd(&lh[bin], 2u);
                         }
                     }
                 }

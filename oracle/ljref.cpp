@@ -643,7 +643,8 @@ double ljref_initial_condition(int n, double density, double temperature, unsign
 void ljref_rdf_hist(int n, const double* pos, const double* L, double rmax, int bins,
                     double* hist)
 {
-    double dr = rmax / bins;
+    // bin = floor(r * (bins / rmax)), the GPU kernel's formula
+    const double inv_dr = (double)bins / rmax;
     int im[3];
     for (int a = 0; a < 3; ++a) im[a] = (int)std::ceil(rmax / L[a]) + 1;
     for (int i = 0; i < n; ++i)
@@ -657,7 +658,7 @@ void ljref_rdf_hist(int n, const double* pos, const double* L, double rmax, int 
                         double dz = (pos[3 * i + 2] - pos[3 * j + 2]) - iz * L[2];
                         double r2 = sqnorm(dx, dy, dz);
                         if (r2 < rmax * rmax) {
-                            int k = (int)(std::sqrt(r2) / dr);
+                            int k = (int)(std::sqrt(r2) * inv_dr);
                             if (k < bins) hist[k] += 1.0;
                         }
                     }
