@@ -1954,6 +1954,14 @@ __global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bin
         for (int a = 0; a < 3; ++a) R[a] = (int)ceil(rmax / S.a[a]);
         const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
         const double rmax2 = rmax * rmax, inv_dr = (double)bins / rmax;
+        // position inside the home cell, for the cell-box distance bound below
+        const double l3[3] = {xi - c[0] * S.a[0], yi - c[1] * S.a[1], zi - c[2] * S.a[2]};
+        // distance from the particle to the box of the cell at offset o along
+        // axis q, less a margin far above the rounding of cell boundaries
+        auto gap = [&](int q, int o) {
+            const double g = o > 0 ? o * S.a[q] - l3[q] : (o < 0 ? l3[q] - (o + 1) * S.a[q] : 0.0);
+            return fmax(g - 1e-9, 0.0);
+        };
         // Half shell: every stencil offset is a distinct (cell, image), and the
         // pair (j, i) at the mirrored offset has exactly the negated separation
         // (IEEE subtraction is sign-symmetric), so it falls in the same bin.
@@ -1963,14 +1971,19 @@ __global__ __launch_bounds__(kBlock) void k_rdf(StepArgs p, double rmax, int bin
             int tx = c[0] + ox;
             int ix = (tx >= 0) ? tx / S.nc[0] : -((-tx + S.nc[0] - 1) / S.nc[0]);
             int nbx = tx - ix * S.nc[0];
+            const double gx = gap(0, ox);
             for (int oy = -R[1]; oy <= R[1]; ++oy) {
                 if (ox < 0 || (ox == 0 && oy < 0)) continue;
+                const double gy = gap(1, oy);
                 int ty = c[1] + oy;
                 int iy = (ty >= 0) ? ty / S.nc[1] : -((-ty + S.nc[1] - 1) / S.nc[1]);
                 int nby = ty - iy * S.nc[1];
                 for (int oz = -R[2]; oz <= R[2]; ++oz) {
                     const bool home = ox == 0 && oy == 0 && oz == 0;
                     if (ox == 0 && oy == 0 && oz < 0) continue;
+                    const double gz = gap(2, oz);
+                    // no particle of that cell can be within rmax
+                    if ((gx * gx + gy * gy) + gz * gz >= rmax2) continue;
                     int tz = c[2] + oz;
                     int iz = (tz >= 0) ? tz / S.nc[2] : -((-tz + S.nc[2] - 1) / S.nc[2]);
                     int nbz = tz - iz * S.nc[2];
