@@ -379,8 +379,11 @@ def test_mixed_energy_drift_matches_fp64():
 
 # ---- g(r) ------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("n,rho", [(500, 1.2), (256, 0.3)])
+@pytest.mark.parametrize("n,rho", [(500, 1.2), (256, 0.3), (32, 1.185)])
 def test_rdf_histogram_matches_brute_force(n, rho):
+    """Bit-exact against the brute-force oracle, including N = 32 at rho 1.185
+    (L = 3.0, one cell per axis: every stencil cell is an image of the home
+    cell, and each particle meets images of itself)."""
     eng = _engine_mod()
     pos, L = perturbed_lattice(n, rho, 0.3, 5)
     e = make(eng, n, L)
