@@ -11,6 +11,8 @@ Other workloads (one JSON line each, for the records in profiles/):
   --workload plain        N=864 (FCC 6^3), rho 0.8, T 1.0, FP64, 1000 steps
   --workload sweep        256 replicas on a 16x16 (T, rho) grid, N = 4 n^3 with
                           n in [5, 20] from the seed; batched in one engine
+                          (--no-fp64 / --no-plain drop the FP64 batched leg /
+                          the plain per-replica FP64 leg, e.g. under a profiler)
   --workload coexistence  N=2,048,000 at rho 0.30, T 0.80, rc 3.0, dt 0.004
   --workload adjoint      N=4,000, 2000 FP64 steps, adjoint gradients
 
