@@ -54,7 +54,7 @@ extern "C" {
 #define LJ_LAW_NONE 2
 
 typedef struct lj_system_desc {
-    int particle_count;       /* N >= 1 */
+    int particle_count;       /* 1 <= N <= 2^30 */
     double box[3];            /* periodic box sides (tools/bounding_box.hpp:30-61) */
     double cutoff;            /* LJ: reduced cutoff rc/sigma; CONSTANT: absolute cutoff */
     double time_delta;        /* dt */
@@ -93,7 +93,11 @@ int lj_device_count(void);
  * for nsys > 1, the per-replica integrators of api::SimulationPool
  * (api/simulation_pool.cpp:93-101).  The reference's preconditions are
  * errors here: box >= cutoff on every axis (engine/particle_pair_filter.cpp:61-64),
- * LJ cutoff^2 > 2^(1/3) (physics/lennard_jones_force.cpp:45). */
+ * LJ cutoff^2 > 2^(1/3) (physics/lennard_jones_force.cpp:45).  Limits of
+ * this build: at most 1023 cells along a box side, and all systems of one
+ * engine together at most 2^30 particle slots (N rounded up to a multiple of
+ * 256 each) and 2^30 cells.  The descriptors are checked before the device
+ * is looked for, so LJ_E_INVALID comes back on a machine without a GPU too. */
 int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_engine** out);
 
 int lj_engine_destroy(lj_engine* e);

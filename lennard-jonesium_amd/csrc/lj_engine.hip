@@ -62,6 +62,7 @@ struct DevBuf {
 };
 
 constexpr int kMaxParts = 64;   // first-level parts per system of the record sums (one wave)
+constexpr int kMaxSlots = 1 << 30;   // particle slots, and cells, of one engine
 
 struct KernelTimer {
     const char* name;
@@ -612,10 +613,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     if (!systems || nsys < 1) return fail(LJ_E_INVALID, "need at least one system");
     if (precision != LJ_PRECISION_FP64 && precision != LJ_PRECISION_MIXED)
         return fail(LJ_E_INVALID, "unknown precision");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-        return fail(LJ_E_HIP, "no HIP device visible");
 
+    // the descriptors are checked first (host only), then the device is looked for
     auto e = new lj_engine();
     e->nsys = nsys;
     e->precision = precision;
@@ -639,6 +638,10 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         SysDev s;
         std::memset(&s, 0, sizeof(s));
         if (d.particle_count < 1) { delete e; return fail(LJ_E_INVALID, "particle_count < 1"); }
+        if (d.particle_count > kMaxSlots) {
+            delete e;
+            return fail(LJ_E_INVALID, "particle_count exceeds 2^30");
+        }
         if (!(d.time_delta > 0)) { delete e; return fail(LJ_E_INVALID, "time_delta must be > 0"); }
         if (d.force_law < 0 || d.force_law > 2) { delete e; return fail(LJ_E_INVALID, "bad force_law"); }
         if (!d.periodic && d.force_law != LJ_LAW_NONE) {
@@ -704,6 +707,12 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             }
         }
         const int slots = round_up(s.n, kBlock);
+        // slot, cell and block indices are 32-bit on the device
+        if ((long long)atom_off + slots > kMaxSlots ||
+            (long long)cell_off + s.ncell + 1 > kMaxSlots) {
+            delete e;
+            return fail(LJ_E_INVALID, "the systems exceed 2^30 particle slots or cells in one engine");
+        }
         s.atom_off = atom_off;
         s.blk_first = blk;
         s.nblk = slots / kBlock;
@@ -745,6 +754,11 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         for (int b = e->sys[k].sblk_first; b < last; ++b) sblk_sys[b] = k;
     }
 
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete e;
+        return fail(LJ_E_HIP, "no HIP device visible");
+    }
     auto cleanup_fail = [&](int rc) {
         lj_engine_destroy(e);
         return rc;

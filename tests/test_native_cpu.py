@@ -35,6 +35,43 @@ def test_engine_create_without_device_fails_cleanly():
         Engine([SystemSpec(10, 5.0)])
 
 
+def _desc(n=100, box=10.0, cutoff=2.5, dt=0.005, law=0):
+    from lennardjonesium import _native
+    d = _native.lj_system_desc()
+    d.particle_count = n
+    d.box[0] = d.box[1] = d.box[2] = box
+    d.cutoff, d.time_delta, d.epsilon, d.sigma, d.strength = cutoff, dt, 1.0, 1.0, 1.0
+    d.force_law, d.periodic = law, 1
+    return d
+
+
+@pytest.mark.parametrize("descs,precision,message", [
+    ([_desc()], 7, "unknown precision"),
+    ([_desc(n=0)], 0, "particle_count < 1"),
+    ([_desc(n=(1 << 30) + 1)], 0, "particle_count exceeds 2^30"),
+    # N rounds up to 2^30 slots, so a second system does not fit
+    ([_desc(n=(1 << 30) - 100), _desc(n=1000)], 1, "exceed 2^30"),
+    ([_desc(dt=0.0)], 0, "time_delta"),
+    ([_desc(cutoff=1.1)], 0, "Cutoff distance is too short"),          # 1.21 < 2^(1/3)
+    ([_desc(box=2.0)], 0, "less than the cutoff distance"),
+    ([_desc(box=2600.0)], 1, "more than 1023 cells"),
+    ([_desc(), _desc(law=1)], 0, "same force law"),
+    ([_desc(), _desc(law=2)], 0, "LJ_LAW_NONE"),
+])
+def test_engine_create_rejects_invalid_descriptors(descs, precision, message):
+    # the descriptors are checked before any device call, so this runs with
+    # or without a GPU and allocates nothing
+    import ctypes
+    from lennardjonesium import _native
+    lib = _native.engine()
+    arr = (_native.lj_system_desc * len(descs))(*descs)
+    out = ctypes.c_void_p(1)
+    rc = lib.lj_engine_create(arr, len(descs), precision, ctypes.byref(out))
+    assert rc == -1                      # LJ_E_INVALID
+    assert out.value is None
+    assert message in lib.lj_last_error().decode()
+
+
 @pytest.mark.parametrize("n,rho,T,seed,lat", [(500, 0.8, 0.5, 117208264, 2), (864, 0.8, 1.0, 5489, 2),
                                               (100, 0.8, 0.3, 5489, 2), (7, 7 / 8, 0.5, 1, 0),
                                               (54, 0.5, 1.2, 99, 1)])
