@@ -53,8 +53,9 @@ def build(verbose: bool = False) -> pathlib.Path:
     host = OUT / "liblj_host.so"
     htmp = OUT / "liblj_host.so.tmp"
     srcs = [str(HERE / "csrc" / "host" / f) for f in HOST_SOURCES]
+    # the host layer sees only the plain-C headers: no HIP include path
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}",
-          "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", *srcs, f"-L{OUT}", "-llj_engine",
+          *srcs, f"-L{OUT}", "-llj_engine",
           "-Wl,-rpath,$ORIGIN", "-o", str(htmp)], verbose, "liblj_host.so")
     os.replace(htmp, host)
     return target
