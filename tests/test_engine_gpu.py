@@ -262,6 +262,36 @@ def test_trajectory_tracks_oracle_and_graphs_are_exact():
     assert rel[:100].max() < 1e-11 and rel.max() < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle"])
+def test_single_particle_drifts_through_the_wrap(precision, kernel):
+    """N = 1 in a box of 3 cells a side: its own images lie beyond the cutoff,
+    so no pair exists and F = PE = W = 0 exactly; the particle drifts at
+    constant velocity and wraps across two faces (x up, y down).  Positions,
+    displacements and times equal the oracle's bit for bit over 40 steps
+    (a 32-step graph and 8 eager steps)."""
+    eng = _engine_mod()
+    _say("single particle", precision, kernel)
+    L = 8.0
+    pos = np.array([[7.9, 0.05, 4.0]])
+    vel = np.array([[1.0, -2.0, 0.5]])
+    e = make(eng, 1, L, precision=precision, pair_kernel=kernel)
+    e.set_state(0, pos, vel)
+    rec = e.step(40)[:, 0, :]
+    out = e.get_state(0)
+    e.close()
+    st = ljref.State(pos=pos, vel=vel)
+    ref = ljref.run(st, L, 0.005, 40)
+    assert st.pos[0, 0] < 1.0 and st.pos[0, 1] > 7.0          # both wraps happened
+    assert np.array_equal(out["pos"], st.pos)
+    assert np.array_equal(out["disp"], st.disp)
+    assert np.array_equal(out["vel"], vel)
+    assert not out["force"].any() and out["pe"] == 0.0 and out["vir"] == 0.0
+    assert not rec[:, 2].any() and not rec[:, 4].any()       # PE, W
+    assert np.array_equal(rec[:, 0], ref[:, 0])               # time
+    np.testing.assert_allclose(rec, ref, rtol=1e-15, atol=0)
+
+
 @pytest.mark.parametrize("kernel", ["tiled", "particle", "split"])
 def test_batched_replicas_match_single_runs(kernel):
     """One engine for three uneven replicas gives each replica's single-engine
