@@ -52,6 +52,27 @@ def pair_flops(c_half, p_half):
     return 8.0 * c_half + 30.0 * p_half
 
 
+def pair_counts(eng, pos, L):
+    """(C_half, P_half) of the reference's half-shell traversal on these
+    positions (engine/particle_pair_filter.cpp:141-191): candidate pairs,
+    i.e. pairs inside a cell plus the pairs of each cell with its 13
+    half-shell neighbour cells, and in-cutoff pairs.  Both come from an FP64
+    engine, whose cell lists and cutoff decisions are the reference's."""
+    n = len(pos)
+    with eng.Engine([eng.SystemSpec(n, L)], precision="fp64") as e:
+        e.set_state(0, pos, np.zeros_like(pos))
+        start, _ = e.cell_list(0)
+        _, nc, _ = e.cell_keys(0)
+        p_half = int(e.neighbor_counts(0).sum()) // 2
+    m = np.diff(start).astype(np.int64).reshape(tuple(int(x) for x in nc))
+    assert min(nc) >= 3, "the 13 half-shell neighbours are distinct cells from 3 cells a side"
+    c_half = int((m * (m - 1) // 2).sum())
+    for d in itertools.product((-1, 0, 1), repeat=3):
+        if d > (0, 0, 0):                     # one of each +/- pair of offsets
+            c_half += int((m * np.roll(m, tuple(-x for x in d), axis=(0, 1, 2))).sum())
+    return c_half, p_half
+
+
 def drift_stats(E, window=100, scale=None):
     """(secular drift |<E>_end - <E>_start| / s over `window`-step windows,
     max |E_t - E_0| / s), with s = |E_0| unless a scale is given (a replica
@@ -150,9 +171,7 @@ def liquid(args, eng):
     log("mixed kernel ms:", json.dumps(ktm))
     st = em.get_state(0)
     em.close()
-    from oracle import ljref
-
-    c_half, p_half = ljref.pair_counts(st["pos"], L, 2.5)
+    c_half, p_half = pair_counts(eng, st["pos"], L)
     flops = pair_flops(c_half, p_half)
     achieved = flops / (ktm["pair"] * 1e-3) / 1e12
     traffic = read_traffic("pair_mixed_256k_traffic.txt")
