@@ -17,7 +17,9 @@
 //     exclusive scan over all cell counts yields every system's slot range.
 //
 // Arithmetic that decides integers (cell keys, the r^2 < rc^2 test) and the
-// periodic wrap is written exactly as the oracle evaluates it; the whole
+// periodic wrap is written exactly as the reference evaluates it
+// (engine/particle_pair_filter.cpp:143-148 keys, :172-187 the r^2 test;
+// engine/boundary_condition.cpp:36-48 the wrap); the whole
 // file is compiled with -ffp-contract=off, and fused multiply-adds appear only
 // where they are written explicitly (mixed-precision force math).
 
