@@ -65,12 +65,20 @@ def pair_counts(eng, pos, L):
         _, nc, _ = e.cell_keys(0)
         p_half = int(e.neighbor_counts(0).sum()) // 2
     m = np.diff(start).astype(np.int64).reshape(tuple(int(x) for x in nc))
-    assert min(nc) >= 3, "the 13 half-shell neighbours are distinct cells from 3 cells a side"
-    c_half = int((m * (m - 1) // 2).sum())
+    return half_shell_candidates(m), p_half
+
+
+def half_shell_candidates(m):
+    """Candidate pairs of the half-shell traversal over a periodic grid of
+    per-cell counts m (x, y, z): pairs inside each cell plus, for each of 13
+    neighbour offsets (one of every +/- pair), the products of neighbouring
+    counts."""
+    assert min(m.shape) >= 3, "the 13 half-shell neighbours are distinct cells from 3 cells a side"
+    c = int((m * (m - 1) // 2).sum())
     for d in itertools.product((-1, 0, 1), repeat=3):
-        if d > (0, 0, 0):                     # one of each +/- pair of offsets
-            c_half += int((m * np.roll(m, tuple(-x for x in d), axis=(0, 1, 2))).sum())
-    return c_half, p_half
+        if d > (0, 0, 0):
+            c += int((m * np.roll(m, tuple(-x for x in d), axis=(0, 1, 2))).sum())
+    return c
 
 
 def drift_stats(E, window=100, scale=None):

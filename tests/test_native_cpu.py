@@ -116,3 +116,22 @@ def test_sweep_sizes_deterministic_and_in_range():
     assert counts.min() > 180 and counts.max() < 340      # roughly uniform over 16 values
     # std::mt19937(5489): first output 3499211612 -> 3499211612 % 16 = 12 -> 17
     assert a[0] == 17
+
+
+@pytest.mark.parametrize("n,rho", [(864, 0.8), (4000, 0.3), (32000, 0.05)])
+def test_bench_candidate_count_matches_oracle(n, rho):
+    # bench.py prices the pair kernel's flops on the half-shell candidate
+    # count it forms from the engine's per-cell counts; the same formula on
+    # the oracle's cells must give the oracle's count
+    import sys
+    sys.path.insert(0, str(ROOT))
+    import bench
+    from oracle import ljref
+    st, L = ljref.initial_condition(n, rho, 1.0, seed=11)
+    pos = ljref.wrap(st.pos + np.random.default_rng(n).uniform(-0.4, 0.4, st.pos.shape), L)
+    nc = int(np.floor(L / 2.5))
+    k = np.clip(np.floor(pos * (nc / L)).astype(int), 0, nc - 1)
+    m = np.zeros((nc, nc, nc), dtype=np.int64)
+    np.add.at(m, (k[:, 0], k[:, 1], k[:, 2]), 1)
+    c_ref, _ = ljref.pair_counts(pos, L, 2.5)
+    assert bench.half_shell_candidates(m) == c_ref
