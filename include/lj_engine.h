@@ -126,6 +126,14 @@ int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double*
  * them again. */
 int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records);
 
+/* Prepares calls of lj_engine_step of up to max_steps steps: sizes the
+ * device record buffer and captures the step graphs now, so that such a call
+ * neither allocates device memory nor captures graphs inside a caller's timed
+ * region.  No reference counterpart (the reference allocates nothing per
+ * step; its per-step measurement goes straight to a message queue,
+ * control/simulation_controller.cpp:60-69). */
+int lj_engine_reserve(lj_engine* e, int max_steps);
+
 /* Recomputes forces, potential energy and virial on the current positions
  * without moving the particles (engine/force_calculation.cpp:46-71).
  * pe / vir receive nsys values each when not NULL. */
@@ -194,7 +202,9 @@ int lj_engine_rdf_read(lj_engine* e, double* counts, int* samples);
  * (rc = cutoff * sigma moves with sigma), the initial positions and
  * velocities (N x 3, particle-id order).  States are checkpointed every
  * `every` steps and recomputed segment by segment in the reverse sweep.
- * The engine is left holding the initial state. */
+ * The engine is left holding the state it had on entry -- positions,
+ * velocities, displacements and elapsed time -- with the forces, potential
+ * energy and virial recomputed on those positions. */
 int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, double* loss,
                       double* grad_eps, double* grad_sigma, double* grad_pos, double* grad_vel);
 

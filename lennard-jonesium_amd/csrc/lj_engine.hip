@@ -114,6 +114,10 @@ struct lj_engine {
     double* d_partial2 = nullptr;   // [nsys][kMaxParts][4] first-level record sums
     double* d_records = nullptr;
     int rec_cap = 0;
+    // record rows the step kernels write: d_records / rec_cap unless an
+    // entry point points them elsewhere for the duration of its call
+    double* rec_buf = nullptr;
+    int rec_rows = 0;
     double* d_measure = nullptr;
     unsigned long long* d_rdf = nullptr;
     int rdf_bins = 0, rdf_samples = 0;
@@ -131,6 +135,7 @@ struct lj_engine {
     hipGraphExec_t graph2 = nullptr;   // two steps starting from buffer set 0
     hipGraphExec_t graphN = nullptr;   // kGraphSteps steps starting from buffer set 0
     int graph_cap = -1;               // record capacity baked into the graph
+    const double* graph_rec = nullptr;   // record buffer baked into the graph
 
     bool profiling = false;
     std::vector<KernelTimer> timers;
@@ -193,10 +198,10 @@ struct lj_engine {
         f.pe = d_pe;
         f.vir = d_vir;
         f.step = d_step;
-        f.records = d_records;
+        f.records = rec_buf ? rec_buf : d_records;
         f.measure = d_measure;
         f.nsys = nsys;
-        f.cap = rec_cap;
+        f.cap = rec_buf ? rec_rows : rec_cap;
         f.mode = mode;
         // a step's sums come from the force kernels (per-system layout); a
         // force-only or measure call's from its own last pair launch
@@ -471,12 +476,24 @@ int enqueue_step(lj_engine* e, bool chain = false)
     return enqueue_pair(e, true, false);
 }
 
+void drop_graphs(lj_engine* e);
+
+// Grows the record buffer to hold nsteps steps.  The step graphs carry the
+// buffer's address in their kernel arguments, so they are destroyed before
+// the old buffer is freed (and captured again on their next use).
 int ensure_records(lj_engine* e, int nsteps)
 {
     if (nsteps <= e->rec_cap) return LJ_OK;
+    if ((long long)nsteps * e->nsys * 7 > (1LL << 40))
+        return fail(LJ_E_INVALID, "records of nsteps steps x systems exceed 8 TiB");
     int cap = std::max(nsteps, 64);
-    if (e->d_records) HIP_TRY(hipFree(e->d_records));
+    drop_graphs(e);
+    if (e->d_records) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        HIP_TRY(hipFree(e->d_records));
+    }
     e->d_records = nullptr;
+    e->rec_cap = 0;
     int rc = dev_alloc(&e->d_records, (size_t)cap * e->nsys * 7);
     if (rc) return rc;
     e->rec_cap = cap;
@@ -526,7 +543,10 @@ void drop_graphs(lj_engine* e)
 
 int build_graph(lj_engine* e)
 {
-    if (e->graph2 && e->graphN && e->graph_cap == e->rec_cap) return LJ_OK;
+    if (e->graph2 && e->graphN && e->graph_cap == e->rec_cap && e->graph_rec == e->d_records &&
+        !e->rec_buf)
+        return LJ_OK;
+    if (e->rec_buf) return fail(LJ_E_STATE, "step graphs are captured on the engine's own records only");
     drop_graphs(e);
     int rc = capture_steps(e, 2, &e->graph2);
     if (!rc) rc = capture_steps(e, kGraphSteps, &e->graphN);
@@ -535,6 +555,7 @@ int build_graph(lj_engine* e)
         return rc;
     }
     e->graph_cap = e->rec_cap;
+    e->graph_rec = e->d_records;
     return LJ_OK;
 }
 
@@ -584,6 +605,8 @@ int check_sys(const lj_engine* e, int sys)
     if (sys < 0 || sys >= e->nsys) return fail(LJ_E_INVALID, "system index out of range");
     return LJ_OK;
 }
+
+int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records);
 
 // Ensures slots are in cell order without moving particles (needed before
 // any traversal when the state was just loaded).
@@ -954,10 +977,38 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
     if (nsteps < 0) return fail(LJ_E_INVALID, "nsteps < 0");
     if (nsteps == 0) return LJ_OK;
     if ((rc = ensure_records(e, nsteps))) return rc;
+    const bool graphs = e->use_graphs && !e->profiling && e->has_cells;
+    return run_steps(e, nsteps, graphs, records);
+}
+
+int lj_engine_reserve(lj_engine* e, int max_steps)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (max_steps < 0) return fail(LJ_E_INVALID, "max_steps < 0");
+    if ((rc = ensure_records(e, max_steps))) return rc;
+    if (e->use_graphs && !e->profiling && e->has_cells && (rc = build_graph(e))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+}  // extern "C"
+
+namespace {
+
+// The step loop shared by the step entry points: nsteps steps of every
+// system, records into the engine's current record buffer (fargs), replayed
+// from the captured step graphs when `graphs` is set, then copied to
+// `records` (optional).
+int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
+{
+    int rc;
+    const ljk::FinalArgs f = e->fargs(0);
+    const long long nrec = (long long)nsteps * e->nsys * 7;
+    hipLaunchKernelGGL(ljk::k_zero, dim3((unsigned)std::min<long long>(4096, (nrec + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, e->stream, f.records, nrec);
     hipLaunchKernelGGL(ljk::k_reset_steps, dim3((e->nsys + 255) / 256), dim3(256), 0, e->stream,
                        e->d_step, e->nsys);
     int done = 0;
-    const bool graphs = e->use_graphs && !e->profiling && e->has_cells;
     if (graphs && nsteps >= 2) {
         if ((rc = build_graph(e))) return rc;
         // the graph starts from buffer set 0: take one eager step if needed
@@ -983,11 +1034,15 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
     if ((rc = enqueue_finalize(e, 0))) return rc;   // the last step's record
     HIP_TRY(hipGetLastError());
     if (records)
-        HIP_TRY(hipMemcpyAsync(records, e->d_records, (size_t)nsteps * e->nsys * sizeof(lj_thermo),
+        HIP_TRY(hipMemcpyAsync(records, f.records, (size_t)nsteps * e->nsys * sizeof(lj_thermo),
                                hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return LJ_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int lj_engine_compute_forces(lj_engine* e, double* pe, double* vir)
 {
@@ -1267,10 +1322,13 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     int* sid = nullptr;
     if ((rc = dev_alloc(&sp.p, 3 * ns)) || (rc = dev_alloc(&sv.p, 3 * ns)) ||
         (rc = dev_alloc(&sd.p, 3 * ns)) || (rc = dev_alloc(&sf.p, 3 * ns)) ||
-        (rc = dev_alloc(&st.p, 3 * (size_t)e->nsys)) || (rc = dev_alloc(&sid, ns)))
+        (rc = dev_alloc(&st.p, 3 * (size_t)e->nsys)) || (rc = dev_alloc(&sid, ns))) {
+        if (sid) (void)hipFree(sid);
         return rc;
+    }
     const int cur0 = e->cur;
-    const bool sorted0 = e->sorted;
+    const int variant0 = e->pair_variant;
+    const double occ0 = e->split_occupancy;
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e->stream);
     };
@@ -1285,9 +1343,14 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         if (h == hipSuccess) h = cp(st.p + 2 * e->nsys, e->d_vir, e->nsys * sizeof(double));
         return h;
     };
+    // Puts the saved state back into buffer set cur0.  The cell ranges, cell
+    // of each slot and mixed-precision coordinates still describe a trial's
+    // last sort, so the state counts as unsorted: the next traversal
+    // (pairs, g(r), neighbour counts) rebuilds the cells first.
     auto restore = [&]() {
         e->cur = cur0;
-        e->sorted = sorted0;
+        e->sorted = false;
+        e->prekicked = false;
         hipError_t h = cp(e->d_pos[cur0], sp.p, 3 * ns * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_vel[cur0], sv.p, 3 * ns * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_disp[cur0], sd.p, 3 * ns * sizeof(double));
@@ -1296,10 +1359,24 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         if (h == hipSuccess) h = cp(e->d_time, st.p, e->nsys * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_pe, st.p + e->nsys, e->nsys * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_vir, st.p + 2 * e->nsys, e->nsys * sizeof(double));
+        if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
         return h;
     };
-    hipError_t he = save();
-    if (he != hipSuccess) { (void)hipFree(sid); return fail(LJ_E_HIP, "autotune: save failed"); }
+    // every exit after the save goes through here: the state and the kernel
+    // policy the engine had on entry are put back unless a winner was chosen
+    auto finish = [&](int code, bool restored) {
+        if (!restored) (void)restore();
+        (void)hipFree(sid);
+        if (code) {
+            const std::string msg = g_err;
+            e->split_occupancy = occ0;
+            e->pair_variant = variant0;
+            (void)assign_pair_kernels(e);
+            g_err = msg;
+        }
+        return code;
+    };
+    if (save() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: save failed"), true);
     // candidates: all tiled, all per-particle, and splits by cell occupancy
     // whose tiled sets differ from those and from each other
     struct Candidate {
@@ -1318,28 +1395,27 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
             cands.push_back({LJ_PAIR_BY_OCCUPANCY, thr});
         }
     }
-    const double occ0 = e->split_occupancy;
     double best = 1e300;
-    Candidate best_c = {e->pair_variant, occ0};
+    Candidate best_c = {variant0, occ0};
     for (const Candidate& c : cands) {
         e->split_occupancy = c.variant == LJ_PAIR_BY_OCCUPANCY ? c.occupancy : occ0;
-        if ((rc = lj_engine_set_pair_kernel(e, c.variant))) break;
-        if ((rc = lj_engine_step(e, 2, nullptr))) break;   // warm-up (and graph build)
-        HIP_TRY(hipEventRecord(e->ev0, e->stream));
-        if ((rc = lj_engine_step(e, steps, nullptr))) break;
-        HIP_TRY(hipEventRecord(e->ev1, e->stream));
-        HIP_TRY(hipEventSynchronize(e->ev1));
+        if ((rc = lj_engine_set_pair_kernel(e, c.variant))) return finish(rc, false);
+        if ((rc = lj_engine_reserve(e, steps))) return finish(rc, false);   // graphs built untimed
+        if ((rc = lj_engine_step(e, 2, nullptr))) return finish(rc, false);   // warm-up
+        hipError_t h = hipEventRecord(e->ev0, e->stream);
+        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event record failed"), false);
+        if ((rc = lj_engine_step(e, steps, nullptr))) return finish(rc, false);
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        h = hipEventRecord(e->ev1, e->stream);
+        if (h == hipSuccess) h = hipEventSynchronize(e->ev1);
+        if (h == hipSuccess) h = hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event timing failed"), false);
         if (ms < best) { best = ms; best_c = c; }
-        if ((he = restore()) != hipSuccess) break;
+        if (restore() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: restore failed"), true);
     }
-    (void)hipFree(sid);
-    if (he != hipSuccess) return fail(LJ_E_HIP, "autotune: restore failed");
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(e->stream));
     e->split_occupancy = best_c.variant == LJ_PAIR_BY_OCCUPANCY ? best_c.occupancy : occ0;
-    if ((rc = lj_engine_set_pair_kernel(e, best_c.variant))) return rc;
+    if ((rc = lj_engine_set_pair_kernel(e, best_c.variant))) return finish(rc, true);
+    (void)finish(LJ_OK, true);
     if (chosen) *chosen = best_c.variant;
     if (ms_per_step) *ms_per_step = best / steps;
     return LJ_OK;
@@ -1394,7 +1470,9 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
     const double cW = density / (3.0 * n);
     const double eps = S.eps, sig = std::sqrt(S.sigma2);
     DevBuf ck_pos, ck_vel, seg_pos, seg_vel, lam_r, lam_v, pend, phi, jphi, gradw, frc, part, acc;
-    if ((rc = dev_alloc(&ck_pos.p, (size_t)nck * n3)) || (rc = dev_alloc(&ck_vel.p, (size_t)nck * n3)) ||
+    DevBuf disp0, time0;
+    if ((rc = dev_alloc(&disp0.p, n3)) || (rc = dev_alloc(&time0.p, 1)) ||
+        (rc = dev_alloc(&ck_pos.p, (size_t)nck * n3)) || (rc = dev_alloc(&ck_vel.p, (size_t)nck * n3)) ||
         (rc = dev_alloc(&seg_pos.p, (size_t)(every + 1) * n3)) ||
         (rc = dev_alloc(&seg_vel.p, (size_t)(every + 1) * n3)) || (rc = dev_alloc(&lam_r.p, n3)) ||
         (rc = dev_alloc(&lam_v.p, n3)) || (rc = dev_alloc(&pend.p, n3)) || (rc = dev_alloc(&phi.p, n3)) ||
@@ -1402,14 +1480,14 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
         (rc = dev_alloc(&part.p, 5 * (size_t)S.nblk)) || (rc = dev_alloc(&acc.p, 2)))
         return rc;
     const dim3 gb(S.nblk), tb(kBlock), gv((n3 + kBlock - 1) / kBlock);
-    auto to_id = [&](double* pos, double* vel) {
+    auto to_id = [&](double* pos, double* vel, double* disp = nullptr) {
         const ljk::StepArgs p = e->args(e->cur);
-        hipLaunchKernelGGL(ljk::k_to_id, gb, tb, 0, e->stream, p, 0, n, pos, vel);
+        hipLaunchKernelGGL(ljk::k_to_id, gb, tb, 0, e->stream, p, 0, n, pos, vel, disp);
     };
-    auto from_id = [&](const double* pos, const double* vel) {
+    auto from_id = [&](const double* pos, const double* vel, const double* disp = nullptr) {
         const ljk::StepArgs p = e->args(e->cur);
         hipLaunchKernelGGL(ljk::k_from_id, dim3((n + kBlock - 1) / kBlock), tb, 0, e->stream, p, 0,
-                           n, pos, vel);
+                           n, pos, vel, disp);
         e->sorted = false;
     };
     auto forces_now = [&]() -> int {
@@ -1431,9 +1509,12 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
         hipLaunchKernelGGL(ljk::k_adjoint_pair, gb, tb, 0, e->stream, a);
     };
     const bool saved_graphs = e->use_graphs;
+    // the displacements and the clock on entry, given back at the end (the
+    // segments below integrate from checkpoints with cleared displacements)
+    HIP_TRY(hipMemcpyAsync(time0.p, e->d_time, sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     // ---- forward: F(r0), then nsteps with checkpoints every `every` steps
     if ((rc = forces_now())) return rc;
-    to_id(ck_pos.p, ck_vel.p);
+    to_id(ck_pos.p, ck_vel.p, disp0.p);
     std::vector<lj_thermo> rec(nsteps);
     for (int t = 0; t < nsteps;) {
         const int k = std::min(every, nsteps - t);
@@ -1488,13 +1569,13 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
     HIP_TRY(hipMemcpy(jp.data(), jphi.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(lv.data(), lam_v.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hacc, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
-    // leave the engine holding the initial state with F(r0)
-    from_id(ck_pos.p, ck_vel.p);
+    // leave the engine holding the initial state (positions, velocities,
+    // displacements, clock) with F(r0)
+    from_id(ck_pos.p, ck_vel.p, disp0.p);
     if ((rc = forces_now())) return rc;
+    HIP_TRY(hipMemcpyAsync(e->d_time, time0.p, sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->use_graphs = saved_graphs;
-    double zero = 0.0;
-    HIP_TRY(hipMemcpy(e->d_time, &zero, sizeof(double), hipMemcpyHostToDevice));
     if (loss) *loss = L;
     if (grad_eps) *grad_eps = hacc[0];
     if (grad_sigma) *grad_sigma = hacc[1];

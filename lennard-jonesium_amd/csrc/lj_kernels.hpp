@@ -17,11 +17,16 @@
 //     exclusive scan over all cell counts yields every system's slot range.
 //
 // Arithmetic that decides integers (cell keys, the r^2 < rc^2 test) and the
-// periodic wrap is written exactly as the reference evaluates it
+// periodic wrap follows the reference's formulas in one fixed evaluation order
 // (engine/particle_pair_filter.cpp:143-148 keys, :172-187 the r^2 test;
-// engine/boundary_condition.cpp:36-48 the wrap); the whole
-// file is compiled with -ffp-contract=off, and fused multiply-adds appear only
-// where they are written explicitly (mixed-precision force math).
+// engine/boundary_condition.cpp:36-48 the wrap): r^2 = (x^2 + y^2) + z^2,
+// the order of the CPU oracle, against which cell keys and cutoff decisions
+// are bit-exact.  The reference binary itself was built with -ffast-math and
+// Eigen's packet reductions, whose summation order is not fixed by its
+// source, so parity with the reference's own outputs is pinned by tolerances
+// on its recorded trajectory, not bit for bit.  The whole file is compiled
+// with -ffp-contract=off, and fused multiply-adds appear only where they are
+// written explicitly (mixed-precision force math).
 
 #pragma once
 
@@ -1883,6 +1888,15 @@ __global__ void k_reset_steps(int* step, int nsys)
     if (k < nsys) step[k] = 0;
 }
 
+// Clears n doubles (the record rows of a call: rows of frozen systems are
+// not written by the steps and read back as zeros).
+__global__ __launch_bounds__(kBlock) void k_zero(double* v, long long n)
+{
+    for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
+         k += (long long)gridDim.x * kBlock)
+        v[k] = 0.0;
+}
+
 // v *= scale for the slots of one system
 __global__ __launch_bounds__(kBlock) void k_scale_velocities(StepArgs p, int sysk, double scale)
 {
@@ -2096,9 +2110,10 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pair(AdjArgs a)
     }
 }
 
-// Copies the state of one system from its slots to id order (pos, vel).
+// Copies the state of one system from its slots to id order (pos, vel, and
+// the displacements when disp is not null).
 __global__ __launch_bounds__(kBlock) void k_to_id(StepArgs p, int sysk, int n, double* pos,
-                                                  double* vel)
+                                                  double* vel, double* disp)
 {
     const SysDev& S = p.sys[sysk];
     const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
@@ -2106,12 +2121,14 @@ __global__ __launch_bounds__(kBlock) void k_to_id(StepArgs p, int sysk, int n, d
     const int i = p.id[s];
     pos[i] = p.pos.x[s]; pos[n + i] = p.pos.y[s]; pos[2 * n + i] = p.pos.z[s];
     vel[i] = p.vel.x[s]; vel[n + i] = p.vel.y[s]; vel[2 * n + i] = p.vel.z[s];
+    if (disp) { disp[i] = p.disp.x[s]; disp[n + i] = p.disp.y[s]; disp[2 * n + i] = p.disp.z[s]; }
 }
 
 // Loads one system from id-ordered arrays into its slots (slot atom_off + i
-// holds particle i); displacements and forces are cleared.
+// holds particle i); displacements come from disp (cleared when it is null),
+// forces are cleared.
 __global__ __launch_bounds__(kBlock) void k_from_id(StepArgs p, int sysk, int n, const double* pos,
-                                                    const double* vel)
+                                                    const double* vel, const double* disp)
 {
     const SysDev& S = p.sys[sysk];
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -2119,7 +2136,11 @@ __global__ __launch_bounds__(kBlock) void k_from_id(StepArgs p, int sysk, int n,
     const int s = S.atom_off + i;
     p.pos.x[s] = pos[i]; p.pos.y[s] = pos[n + i]; p.pos.z[s] = pos[2 * n + i];
     p.vel.x[s] = vel[i]; p.vel.y[s] = vel[n + i]; p.vel.z[s] = vel[2 * n + i];
-    p.disp.x[s] = 0; p.disp.y[s] = 0; p.disp.z[s] = 0;
+    if (disp) {
+        p.disp.x[s] = disp[i]; p.disp.y[s] = disp[n + i]; p.disp.z[s] = disp[2 * n + i];
+    } else {
+        p.disp.x[s] = 0; p.disp.y[s] = 0; p.disp.z[s] = 0;
+    }
     p.force.x[s] = 0; p.force.y[s] = 0; p.force.z[s] = 0;
     p.id[s] = i;
 }

@@ -132,6 +132,11 @@ class Engine:
         check(self.lib.lj_engine_step(self.h, int(nsteps), ptr))
         return rec
 
+    def reserve(self, max_steps):
+        """Sizes the record buffer and captures the step graphs for calls of
+        up to max_steps steps, so such calls allocate and capture nothing."""
+        check(self.lib.lj_engine_reserve(self.h, int(max_steps)))
+
     def compute_forces(self):
         pe = np.zeros(self.nsys)
         w = np.zeros(self.nsys)

@@ -447,6 +447,17 @@ void ljref_pair_counts(int n, const double* pos, const double* L, double rc, lon
     *inrange = p;
 }
 
+// In-cutoff neighbours of every particle (each pair counted for both of its
+// particles, every image separately) of the reference's cell filter.
+void ljref_neighbor_counts(int n, const double* pos, const double* L, double rc, int* counts)
+{
+    std::fill(counts, counts + n, 0);
+    for_each_pair_cell(n, pos, L, rc, [&](int i, int j, const double*, const int*) {
+        ++counts[i];
+        ++counts[j];
+    });
+}
+
 // ---- forces --------------------------------------------------------------
 
 // law: 0 = LJ (rc reduced, eps, sigma), 1 = constant mock (strength, rc),
@@ -662,6 +673,59 @@ void ljref_rdf_hist(int n, const double* pos, const double* L, double rmax, int 
                             if (k < bins) hist[k] += 1.0;
                         }
                     }
+}
+
+// The histogram of ljref_rdf_hist computed over a cell grid of side >= rmax
+// (three or more cells per axis, else the brute force above).  Each (j,
+// image) within rmax of particle i lies in one of the 27 cells around i's
+// cell, and with three or more cells per axis those 27 (cell, image)
+// combinations are distinct, so every ordered pair and image is counted once
+// with the brute force's arithmetic, (r_i - r_j) - image * L.  Positions must
+// lie in [0, L).
+void ljref_rdf_cells(int n, const double* pos, const double* L, double rmax, int bins,
+                     double* hist)
+{
+    int nc[3];
+    for (int a = 0; a < 3; ++a) nc[a] = (int)std::floor(L[a] / rmax);
+    if (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) {
+        ljref_rdf_hist(n, pos, L, rmax, bins, hist);
+        return;
+    }
+    CellList cl;
+    build_cells(n, pos, L, rmax, cl);
+    const double inv_dr = (double)bins / rmax;
+    for (int cx = 0; cx < nc[0]; ++cx)
+        for (int cy = 0; cy < nc[1]; ++cy)
+            for (int cz = 0; cz < nc[2]; ++cz) {
+                const int home = (cx * nc[1] + cy) * nc[2] + cz;
+                for (int ox = -1; ox <= 1; ++ox)
+                    for (int oy = -1; oy <= 1; ++oy)
+                        for (int oz = -1; oz <= 1; ++oz) {
+                            const int t[3] = {cx + ox, cy + oy, cz + oz};
+                            int img[3], nb[3];
+                            for (int a = 0; a < 3; ++a) {
+                                img[a] = t[a] < 0 ? -1 : (t[a] >= nc[a] ? 1 : 0);
+                                nb[a] = t[a] - img[a] * nc[a];
+                            }
+                            const int other = (nb[0] * nc[1] + nb[1]) * nc[2] + nb[2];
+                            for (int a = cl.start[home]; a < cl.start[home + 1]; ++a) {
+                                const int i = cl.atoms[a];
+                                for (int b = cl.start[other]; b < cl.start[other + 1]; ++b) {
+                                    const int j = cl.atoms[b];
+                                    if (i == j && img[0] == 0 && img[1] == 0 && img[2] == 0)
+                                        continue;
+                                    double dx = (pos[3 * i] - pos[3 * j]) - img[0] * L[0];
+                                    double dy = (pos[3 * i + 1] - pos[3 * j + 1]) - img[1] * L[1];
+                                    double dz = (pos[3 * i + 2] - pos[3 * j + 2]) - img[2] * L[2];
+                                    double r2 = sqnorm(dx, dy, dz);
+                                    if (r2 < rmax * rmax) {
+                                        int k = (int)(std::sqrt(r2) * inv_dr);
+                                        if (k < bins) hist[k] += 1.0;
+                                    }
+                                }
+                            }
+                        }
+            }
 }
 
 }  // extern "C"

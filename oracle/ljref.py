@@ -277,6 +277,24 @@ def rdf_hist(pos, L, rmax=4.0, bins=200):
     return hist
 
 
+def rdf_cells(pos, L, rmax=4.0, bins=200):
+    """rdf_hist's histogram over a cell grid of side >= rmax (for large N;
+    the brute force when a box holds fewer than three such cells per axis)."""
+    pos, L = _pos(pos), _box(L)
+    hist = np.zeros(bins)
+    lib().ljref_rdf_cells(len(pos), _d(pos), _d(L), ctypes.c_double(rmax), bins, _d(hist))
+    return hist
+
+
+def neighbor_counts(pos, L, rc):
+    """In-cutoff neighbours of each particle under the reference's cell filter
+    (each pair counted for both particles, every image separately)."""
+    pos, L = _pos(pos), _box(L)
+    out = np.zeros(len(pos), dtype=np.int32)
+    lib().ljref_neighbor_counts(len(pos), _d(pos), _d(L), ctypes.c_double(rc), _i(out))
+    return out
+
+
 # ---- analyzers (physics/analyzers.cpp:44-159, tools/moving_sample.hpp:81-189) ----
 
 def observation(temperatures, energies, virials, times, msds, density, n):

@@ -87,3 +87,35 @@ def test_torch_autograd_function_wraps_the_adjoint():
     assert eps.grad.item() == pytest.approx(2 * ge, rel=1e-12)
     assert sig.grad.item() == pytest.approx(2 * gs, rel=1e-12)
     np.testing.assert_allclose(r0.grad.numpy(), 2 * gr, rtol=1e-12, atol=1e-14)
+
+
+def test_adjoint_mid_run_keeps_displacements_and_time():
+    """Called after steps, the adjoint leaves the engine with the state it
+    found -- positions, velocities, displacements, elapsed time -- and its
+    gradients equal those of a fresh engine loaded with that state (the loss
+    does not depend on displacements or time)."""
+    eng = _engine()
+    pos, vel, L = eng.initial_condition(500, 0.8, 1.0, seed=4)
+    with eng.Engine([eng.SystemSpec(500, L)]) as e:
+        e.set_state(0, pos, vel)
+        e.step(13)
+        before = e.get_state(0)
+        assert before["time"] > 0 and np.abs(before["disp"]).max() > 0
+        got = e.adjoint(30, 0.8, every=10)
+        after = e.get_state(0)
+        rec = e.step(5)
+    for key in ("pos", "vel", "disp"):
+        assert np.array_equal(after[key], before[key]), key
+    assert after["time"] == before["time"]
+    with eng.Engine([eng.SystemSpec(500, L)]) as f:
+        f.set_state(0, before["pos"], before["vel"])
+        want = f.adjoint(30, 0.8, every=10)
+    assert got[0] == want[0] and got[1] == want[1] and got[2] == want[2]
+    assert np.array_equal(got[3], want[3])
+    # the run continues from the kept clock and displacements
+    st = ljref.State(pos=before["pos"], vel=before["vel"])
+    st.disp = before["disp"].copy()
+    st.time = before["time"]
+    st.F, st.pe, st.vir = ljref.forces(before["pos"], L)
+    ref = ljref.run(st, L, 0.005, 5)
+    assert np.abs(rec[:, 0, :] - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -444,6 +444,74 @@ def test_autotune_leaves_no_trace():
     assert np.array_equal(ref.step(30), tuned.step(30))
 
 
+def test_autotune_after_steps_leaves_valid_cell_lists():
+    """Tuning after steps (the slots already sorted) must not leave the cell
+    ranges of a trial's last sort behind: the pair set and g(r) read
+    afterwards equal those of a fresh engine on the same state."""
+    eng = _engine_mod()
+    pos, vel, L = eng.initial_condition(2048, 0.7, 1.0, seed=3)
+    spec = [eng.SystemSpec(2048, L)]
+    e = eng.Engine(spec)
+    e.set_state(0, pos, vel)
+    e.step(7)
+    st = e.get_state(0)
+    e.autotune(4)
+    got_pairs = _pairset(*e.pairs(0))
+    e.rdf_sample(4.0, 200)
+    got_rdf, _ = e.rdf_read(200)
+    after = e.get_state(0)
+    e.close()
+    for key in ("pos", "vel", "disp", "force"):
+        assert np.array_equal(after[key], st[key]), key
+    with eng.Engine(spec) as f:
+        f.set_state(0, st["pos"], st["vel"], st["disp"], st["force"], st["time"])
+        assert got_pairs == _pairset(*f.pairs(0))
+        f.rdf_sample(4.0, 200)
+        want_rdf, _ = f.rdf_read(200)
+    assert np.array_equal(got_rdf, want_rdf)
+    assert np.array_equal(got_rdf[0], ljref.rdf_hist(st["pos"], L, 4.0, 200))
+
+
+def test_records_buffer_growth_keeps_graphs_valid():
+    """A call longer than any before grows the record buffer; the step graphs
+    captured on the old buffer are dropped first and the records stay exact
+    (one engine growing 10 -> 70 -> 300 steps against one reserved up front)."""
+    eng = _engine_mod()
+    pos, vel, L = eng.initial_condition(864, 0.8, 1.0, seed=9)
+    a = eng.Engine([eng.SystemSpec(864, L)])
+    b = eng.Engine([eng.SystemSpec(864, L)])
+    b.reserve(380)
+    for e in (a, b):
+        e.set_state(0, pos, vel)
+    ra = np.concatenate([a.step(k) for k in (10, 70, 300)])
+    rb = np.concatenate([b.step(k) for k in (10, 70, 300)])
+    a.close()
+    b.close()
+    assert np.array_equal(ra, rb)
+    st = ljref.State(pos=pos, vel=vel)
+    ref = ljref.run(st, L, 0.005, 380)
+    assert (np.abs(ra - ref) / np.abs(ref))[:200].max() < 1e-10
+
+
+def test_frozen_replica_records_read_zero():
+    """Rows of a frozen system are not written by the steps; the call clears
+    them, so they read as zeros (not the previous call's rows)."""
+    eng = _engine_mod()
+    specs, states = [], []
+    for k, n in enumerate((500, 864)):
+        pos, vel, L = eng.initial_condition(n, 0.8, 1.0, seed=k)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((pos, vel))
+    with eng.Engine(specs) as e:
+        for k, (p, v) in enumerate(states):
+            e.set_state(k, p, v)
+        first = e.step(6)
+        e.set_active(1, False)
+        second = e.step(6)
+    assert np.all(first[:, 1, :] != 0)
+    assert np.all(second[:, 1, :] == 0) and np.all(second[:, 0, 0] > first[-1, 0, 0])
+
+
 @pytest.mark.parametrize("kernel", ["tiled", "particle"])
 def test_clumped_cell_chunks_and_stage_splits(kernel):
     """300 particles inside one cell of a 4^3 grid: the tiled kernel walks

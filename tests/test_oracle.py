@@ -295,3 +295,30 @@ def test_rdf_fcc_shells():
         k = int(shell / (1.3 * a / 1300))
         assert hist[max(k - 2, 0):k + 3].sum() == n * count
     assert hist.sum() == n * (12 + 6 + 24)
+
+
+@pytest.mark.parametrize("n,L,rmax", [(400, 14.0, 4.0), (300, [12.5, 16.0, 13.1], 4.0),
+                                      (200, 9.0, 2.9), (150, 10.0, 4.0)])
+def test_rdf_cells_equals_brute_force(n, L, rmax):
+    """The cell-grid histogram (used for replicas too large for the brute
+    force) counts exactly what the brute force counts, with particles placed
+    on and next to cell faces (L = 10, rmax = 4: two cells per axis, so the
+    brute force itself runs)."""
+    rng = np.random.default_rng(n)
+    Lv = np.broadcast_to(np.asarray(L, float), 3)
+    pos = rng.uniform(0, 1, size=(n, 3)) * Lv
+    a = Lv / np.floor(Lv / rmax)
+    pos[:6] = [a, np.nextafter(a, 0), np.nextafter(a, 2 * a), 2 * a, np.zeros(3),
+               np.nextafter(Lv, 0)]
+    ref = ljref.rdf_hist(pos, Lv, rmax, 100)
+    got = ljref.rdf_cells(pos, Lv, rmax, 100)
+    assert np.array_equal(got, ref) and ref.sum() > 0
+
+
+@pytest.mark.parametrize("n,L,rc", [(60, 5.0, 2.5), (40, 3.0, 2.5), (500, 9.0, 2.5)])
+def test_neighbor_counts_equal_pair_set(n, L, rc):
+    """Per-particle counts equal the pair set's, each pair counted for both."""
+    pos = np.random.default_rng(n).uniform(0, L, size=(n, 3))
+    idx, _ = ljref.pairs_naive(pos, L, rc)
+    want = np.bincount(idx[:, 0], minlength=n) + np.bincount(idx[:, 1], minlength=n)
+    assert np.array_equal(ljref.neighbor_counts(pos, L, rc), want)
