@@ -61,7 +61,6 @@ struct DevBuf {
     }
 };
 
-constexpr int kMaxParts = 64;   // first-level parts per system of the record sums (one wave)
 constexpr int kMaxSlots = 1 << 30;   // particle slots, and cells, of one engine
 
 struct KernelTimer {
@@ -212,7 +211,7 @@ struct lj_engine {
         int nbmax = 1;
         for (const SysDev& s : sys)
             nbmax = std::max(nbmax, (f.cell_blocks && s.tiled) ? s.ncblk : s.nblk);
-        f.nf = std::min(kMaxParts, (nbmax + kBlock - 1) / kBlock);
+        f.nf = std::min(ljk::kMaxParts, (nbmax + kBlock - 1) / kBlock);
         return f;
     }
 
@@ -801,7 +800,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
         (rc = dev_alloc(&e->d_pending, nsys)) ||
-        (rc = dev_alloc(&e->d_partial2, (size_t)nsys * kMaxParts * 4)) ||
+        (rc = dev_alloc(&e->d_partial2, (size_t)nsys * ljk::kMaxParts * 4)) ||
         (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
         return cleanup_fail(rc);
     for (int b = 0; b < 2; ++b) {

@@ -40,6 +40,7 @@ constexpr int kScanTile = 1024;    // cells per block in the scan kernels
 constexpr int kCellsPerBlockMixed = 4;  // home cells (one wave64 each) per block, mixed tiled kernel
 constexpr int kCellsPerBlockFp64 = 1;   // FP64 tiled kernel: single-wave blocks retire independently
 constexpr int kHistBins = 1024;    // LDS histogram window of the cell count (cells)
+constexpr int kMaxParts = 64;      // first-level parts per system of the record sums
 
 enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
 enum Prec { FP64 = 0, MIXED = 1 };
@@ -251,12 +252,13 @@ struct FinalArgs {
     int cell_blocks;   // 1: a system with tiled pair forces has cell-space sums; 0: all particle-space
     int* pending;      // per system: set by a step's pair kernel, cleared here
     double* partial2;  // [nsys][nf] first-level sums (see finalize_part)
-    int nf;            // first-level parts per system
+    int nf;            // first-level part slots per system: the most any system uses
 };
 
 template <int NT>
 __device__ void finalize_range(const FinalArgs& f, int k, const double* part, int nb);
 __device__ void finalize_part(const FinalArgs& f, int k, int r);
+__device__ int parts_of(const FinalArgs& f, int k);
 
 // ---------------------------------------------------------------------------
 // 1. kick + drift + wrap + cell key + cell count
@@ -460,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(StepArgs p, FinalArgs f)
     const int ntile = (p.ncell_total + kScanTile - 1) / kScanTile;
     if ((int)blockIdx.x >= ntile) {
         const int k = blockIdx.x - ntile;
-        finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
+        finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, parts_of(f, k));
         return;
     }
     __shared__ int s_off;
@@ -510,7 +512,7 @@ __global__ __launch_bounds__(kScanOneThreads) void k_scan_one(StepArgs p, FinalA
     if (blockIdx.x > 0) {
         // at most 64 parts: all in wave 0, the same sums as a kBlock reduction
         const int k = blockIdx.x - 1;
-        finalize_range<kScanOneThreads>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
+        finalize_range<kScanOneThreads>(f, k, f.partial2 + 4 * (size_t)k * f.nf, parts_of(f, k));
         return;
     }
     constexpr int NW = kScanOneThreads / 64;
@@ -1846,13 +1848,26 @@ __device__ __forceinline__ const double* block_sums(const FinalArgs& f, const Sy
     return cells ? f.partial_c + 4 * (size_t)S.cblk_first : f.partial + 4 * (size_t)S.blk_first;
 }
 
-// First level: part r of f.nf contiguous parts of system k's block sums,
-// summed by one kBlock-thread block into f.partial2.
-__device__ void finalize_part(const FinalArgs& f, int k, int r)
+// First-level parts of system k's record sums: about one block sum per
+// thread, from system k's own block count only, so that a system's records
+// do not depend on the other systems of its engine.
+__device__ int parts_of(const FinalArgs& f, int k)
 {
     int nb;
+    (void)block_sums(f, f.sys[k], nb);
+    return max(1, min(kMaxParts, (nb + kBlock - 1) / kBlock));
+}
+
+// First level: part r of system k's parts_of(f, k) contiguous parts of its
+// block sums, summed by one kBlock-thread block into f.partial2 (blocks with
+// r beyond the system's parts do nothing).
+__device__ void finalize_part(const FinalArgs& f, int k, int r)
+{
+    const int nfk = parts_of(f, k);
+    if (r >= nfk) return;
+    int nb;
     const double* part = block_sums(f, f.sys[k], nb);
-    const int chunk = (nb + f.nf - 1) / f.nf;
+    const int chunk = (nb + nfk - 1) / nfk;
     const int lo = min(nb, r * chunk), hi = min(nb, lo + chunk);
     double acc[4] = {0, 0, 0, 0};
 #pragma unroll 8
@@ -1879,7 +1894,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize_parts(FinalArgs f)
 __global__ __launch_bounds__(kBlock) void k_finalize_reduced(FinalArgs f)
 {
     const int k = blockIdx.x;
-    finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, f.nf);
+    finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, parts_of(f, k));
 }
 
 __global__ void k_reset_steps(int* step, int nsys)

@@ -490,7 +490,28 @@ def test_records_buffer_growth_keeps_graphs_valid():
     assert np.array_equal(ra, rb)
     st = ljref.State(pos=pos, vel=vel)
     ref = ljref.run(st, L, 0.005, 380)
-    assert (np.abs(ra - ref) / np.abs(ref))[:200].max() < 1e-10
+    assert (np.abs(ra[:, 0, :] - ref) / np.abs(ref))[:200].max() < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_small_replica_beside_a_large_one_matches_single_run(precision):
+    """The record sums of a system are split into first-level parts by its
+    own block count: a 500-particle replica batched with a 256,000-particle
+    one (64 parts) keeps the records of its single run (one part) bit for bit."""
+    eng = _engine_mod()
+    specs, states = [], []
+    for k, (n, rho) in enumerate([(256000, 0.8442), (500, 0.8)]):
+        pos, vel, L = eng.initial_condition(n, rho, 0.72, seed=k)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((pos, vel))
+    with eng.Engine(specs, precision=precision) as e:
+        for k, (p, v) in enumerate(states):
+            e.set_state(k, p, v)
+        rec = e.step(20)
+    for k in range(2):
+        with eng.Engine([specs[k]], precision=precision) as s:
+            s.set_state(0, *states[k])
+            assert np.array_equal(rec[:, k, :], s.step(20)[:, 0, :]), k
 
 
 def test_frozen_replica_records_read_zero():

@@ -135,3 +135,15 @@ def test_bench_candidate_count_matches_oracle(n, rho):
     np.add.at(m, (k[:, 0], k[:, 1], k[:, 2]), 1)
     c_ref, _ = ljref.pair_counts(pos, L, 2.5)
     assert bench.half_shell_candidates(m) == c_ref
+
+
+def test_bench_energy_stats_known_answer():
+    """bench.py's conservation figures on a hand-made series."""
+    import bench
+    E = np.array([-10.0, -10.001, -9.999, -10.0005])
+    got = bench.energy_stats(E)
+    assert got["steps"] == 4 and got["window"] == 1
+    assert got["endpoint"] == pytest.approx(5e-5, rel=1e-9)
+    assert got["max_dev"] == pytest.approx(1e-4, rel=1e-9)
+    assert got["windowed"] == pytest.approx(5e-5, rel=1e-9)
+    assert bench.energy_stats(E, scale=20.0)["max_dev"] == pytest.approx(5e-5, rel=1e-9)
