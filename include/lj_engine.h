@@ -126,6 +126,19 @@ int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double*
  * them again. */
 int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records);
 
+/* lj_engine_step without any allocation: no hipMalloc / hipFree, no graph
+ * capture or instantiation, no library routine that allocates.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture elsewhere in the process,
+ * tight serving loops, harnesses that forbid allocation).  The records are
+ * staged in a fixed device array of LJ_NOALLOC_MAX_ROWS rows (one lj_thermo
+ * per step and system) that every engine of a device shares: nsteps *
+ * nsys beyond that is LJ_E_INVALID; profiling must be off (LJ_E_STATE).
+ * Results, records and state equal lj_engine_step's bit for bit.  Calls on
+ * engines of the same device that overlap in time race on that array. */
+#define LJ_NOALLOC_MAX_ROWS 16384
+int lj_engine_step_noalloc(lj_engine* e, int nsteps, lj_thermo* records);
+
 /* Prepares calls of lj_engine_step of up to max_steps steps: sizes the
  * device record buffer and captures the step graphs now, so that such a call
  * neither allocates device memory nor captures graphs inside a caller's timed

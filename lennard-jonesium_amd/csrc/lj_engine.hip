@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,39 @@ struct KernelTimer {
     long long calls = 0;
 };
 
+// Record rows (one lj_thermo per step and system) of lj_engine_step_noalloc,
+// which allocates nothing: a fixed array on every device the library runs on,
+// shared by all engines of that device and kept until the process ends.
+constexpr int kScratchRows = LJ_NOALLOC_MAX_ROWS;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchRows * 7];
+
+namespace {
+
+// Device address of `scratch` on each device, taken once per device (with
+// that device current) by hipGetSymbolAddress: the array's name in host code
+// is a host-side stand-in, not a device address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_address(int dev, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return fail(LJ_E_HIP, "device index beyond 64");
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t he = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (he != hipSuccess || !ptr)
+            return fail(LJ_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(he));
+        g_scratch[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return LJ_OK;
+}
+
 }  // namespace
 
 struct lj_engine {
@@ -117,6 +151,8 @@ struct lj_engine {
     // entry point points them elsewhere for the duration of its call
     double* rec_buf = nullptr;
     int rec_rows = 0;
+    int device = 0;               // the HIP device current at create
+    double* scratch = nullptr;    // that device's address of the scratch record rows
     double* d_measure = nullptr;
     unsigned long long* d_rdf = nullptr;
     int rdf_bins = 0, rdf_samples = 0;
@@ -786,6 +822,9 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         return rc;
     };
     int rc;
+    if (hipGetDevice(&e->device) != hipSuccess)
+        return cleanup_fail(fail(LJ_E_HIP, "hipGetDevice failed"));
+    if ((rc = scratch_address(e->device, &e->scratch))) return cleanup_fail(rc);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(LJ_E_HIP, "hipStreamCreate failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
@@ -978,6 +1017,30 @@ int lj_engine_step(lj_engine* e, int nsteps, lj_thermo* records)
     if ((rc = ensure_records(e, nsteps))) return rc;
     const bool graphs = e->use_graphs && !e->profiling && e->has_cells;
     return run_steps(e, nsteps, graphs, records);
+}
+
+int lj_engine_step_noalloc(lj_engine* e, int nsteps, lj_thermo* records)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(LJ_E_INVALID, "nsteps < 0");
+    if (nsteps == 0) return LJ_OK;
+    if ((long long)nsteps * e->nsys > kScratchRows)
+        return fail(LJ_E_INVALID, "lj_engine_step_noalloc: nsteps * systems = " +
+                                      std::to_string((long long)nsteps * e->nsys) +
+                                      " exceeds the " + std::to_string(kScratchRows) +
+                                      " record rows of its scratch array");
+    if (e->profiling)
+        return fail(LJ_E_STATE, "lj_engine_step_noalloc: switch profiling off first "
+                                "(profiling creates timing events)");
+    // the step kernels write their records into this device's scratch rows
+    // for the duration of the call; eager launches (no graph capture)
+    e->rec_buf = e->scratch;
+    e->rec_rows = kScratchRows / e->nsys;
+    rc = run_steps(e, nsteps, false, records);
+    e->rec_buf = nullptr;
+    e->rec_rows = 0;
+    return rc;
 }
 
 int lj_engine_reserve(lj_engine* e, int max_steps)

@@ -71,6 +71,7 @@ def engine():
         "lj_engine_get_state": (c_i, [V, c_i, P_d, P_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_step": (c_i, [V, c_i, ctypes.POINTER(lj_thermo)]),
         "lj_engine_reserve": (c_i, [V, c_i]),
+        "lj_engine_step_noalloc": (c_i, [V, c_i, ctypes.POINTER(lj_thermo)]),
         "lj_engine_compute_forces": (c_i, [V, P_d, P_d]),
         "lj_engine_set_temperature": (c_i, [V, c_i, c_d, P_d]),
         "lj_engine_measure": (c_i, [V, ctypes.POINTER(lj_thermo)]),
