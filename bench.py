@@ -159,6 +159,14 @@ def energy_gate(eng, specs, states, precision, pair_kernel="tiled"):
     return out
 
 
+PER_CALL_KERNELS = ("finalize",)   # run once per call, not once per step
+
+
+def step_kernel_ms(kt):
+    """Sum of the per-step kernels' mean durations (ms)."""
+    return sum(v for k, v in kt.items() if k not in PER_CALL_KERNELS)
+
+
 def kernel_times(e, steps=40):
     """Mean duration of each step kernel (HIP events on the engine's stream)."""
     e.set_profiling(True)
@@ -302,7 +310,8 @@ def liquid(args, eng):
                              "frac": a64 / PEAK["fp64"], "kernel_ms": kt64["pair"],
                              "step": step_roofline(flops, 200.0 * n, res64["ms_per_step"],
                                                    PEAK["fp64"])}
-        res64["launch_gaps_us_per_step"] = 1e3 * (res64["ms_per_step"] - sum(kt64.values()))
+        res64["launch_gaps_us_per_step"] = 1e3 * (gates["fp64"]["ms_per_step_10000_steps"] -
+                                                  step_kernel_ms(kt64)) if "fp64" in gates else None
     cpu = None
     if not args.no_cpu:
         rate_cpu, dt_cpu = cpu_baseline(n, rho, T, seed, args.cpu_steps)
@@ -330,7 +339,11 @@ def liquid(args, eng):
         "force_rms_rel_err_vs_fp64": frms,
         "roofline": roof,
         "kernel_ms": ktm,
-        "launch_gaps_us_per_step": 1e3 * (ms_m - sum(ktm.values())),
+        # time per step of the 10^4-step graph replay less the per-step kernels'
+        # event-timed durations (which read 2-5% high): the most a single
+        # persistent launch could save
+        "launch_gaps_us_per_step": (1e3 * (gate_m["ms_per_step_10000_steps"] - step_kernel_ms(ktm))
+                                    if gate_m else None),
         "cpu_baseline": cpu,
     }
 
@@ -378,14 +391,22 @@ def sweep_leg(e, states, steps, warmup, nsys):
     return np.concatenate(recs), counts, ns, t
 
 
-def observables(rec, counts, meta):
+def observables(rec, counts, meta, blocks=10):
     """Per-replica time means: PE per particle and the virial pressure
-    rho (<T> + <W> / 3N) (physics/analyzers.cpp:73-75), and g(r) counts."""
+    rho (<T> + <W> / 3N) (physics/analyzers.cpp:73-75), their standard errors
+    from `blocks` block averages, and the g(r) counts."""
     Ns = np.array([m[2] for m in meta], dtype=np.float64)
     rhos = np.array([m[1] for m in meta])
-    pe = rec[:, :, 2].mean(0) / Ns
-    P = rhos * (rec[:, :, 5].mean(0) + rec[:, :, 4].mean(0) / (3.0 * Ns))
-    return pe, P, counts
+    pe_t = rec[:, :, 2] / Ns
+    P_t = rhos * (rec[:, :, 5] + rec[:, :, 4] / (3.0 * Ns))
+    n = (len(rec) // blocks) * blocks
+
+    def sem(x):
+        b = x[:n].reshape(blocks, -1, x.shape[1]).mean(1)
+        return b.std(0, ddof=1) / np.sqrt(blocks)
+
+    return {"pe": pe_t.mean(0), "P": P_t.mean(0), "pe_sem": sem(pe_t), "P_sem": sem(P_t),
+            "rdf": counts}
 
 
 def sweep(args, eng):
@@ -449,20 +470,25 @@ def sweep(args, eng):
         log(f"plain per-replica fp64: {plain_rate:.4e} atom-steps/s over all {len(specs)} replicas")
         po = observables(np.stack(recs, axis=1), np.stack(cnts), meta)
         accept = {}
-        for prec, (pe, P, counts) in obs.items():
-            gr_l1 = np.abs(counts - po[2]).sum(1) / np.maximum(po[2].sum(1), 1)
-            accept[prec] = {"max_abs_rel_diff_mean_pe": float(np.max(np.abs(pe - po[0]) /
-                                                                      np.maximum(np.abs(po[0]),
-                                                                                 1e-12))),
-                            "max_abs_rel_diff_pressure": float(np.max(np.abs(P - po[1]) /
-                                                                      np.maximum(np.abs(po[1]),
-                                                                                 1e-12))),
-                            "max_rdf_l1_rel": float(gr_l1.max()),
-                            "identical": bool(np.array_equal(pe, po[0]) and np.array_equal(P, po[1])
-                                              and np.array_equal(counts, po[2]))}
-        accept["criterion"] = ("fp64 batched: identical to the per-replica runs (same "
-                               "trajectories); mixed batched: the same observables within "
-                               "the statistical spread of chaotic trajectories")
+        for prec, ob in obs.items():
+            gr_l1 = np.abs(ob["rdf"] - po["rdf"]).sum(1) / np.maximum(po["rdf"].sum(1), 1)
+            z = {}
+            for key in ("pe", "P"):
+                err = np.sqrt(ob[key + "_sem"] ** 2 + po[key + "_sem"] ** 2)
+                z[key] = np.abs(ob[key] - po[key]) / np.maximum(err, 1e-300)
+            accept[prec] = {
+                "identical": bool(all(np.array_equal(ob[k], po[k]) for k in ("pe", "P", "rdf"))),
+                "z_mean_pe": {"median": float(np.median(z["pe"])), "max": float(z["pe"].max()),
+                              "replicas_within_3": int((z["pe"] <= 3).sum())},
+                "z_pressure": {"median": float(np.median(z["P"])), "max": float(z["P"].max()),
+                               "replicas_within_3": int((z["P"] <= 3).sum())},
+                "rdf_l1_rel": {"median": float(np.median(gr_l1)), "max": float(gr_l1.max())}}
+        accept["criterion"] = (
+            "fp64 batched: identical to the per-replica runs (same trajectories, bit for bit); "
+            "mixed batched: |mean_batched - mean_plain| / sqrt(sem_b^2 + sem_p^2) per replica "
+            "(standard errors from 10 block averages of each run; chaotic trajectories "
+            "diverge, so the means agree statistically), and the relative L1 distance "
+            "of the g(r) histograms")
     return {"metric": METRIC, "value": out["mixed"]["value"], "unit": "atom-steps/s",
             "n_gpus": 1, "steps": steps, "warmup": args.warmup, "higher_is_better": True,
             "dtype": "f32 pair forces + f64 state/sums", "data": "synthetic",
@@ -500,6 +526,11 @@ def coexistence(args, eng):
         rec, t = timed_run(e, nst, 0, [(pos, vel)])
         E = rec[:, 0, 3]
         alt = rec[0, 0, 1] + abs(rec[0, 0, 2])
+        late = None
+        if nst >= 2 * GATE_STEPS:
+            # after the quench transient: the last 10^4 steps
+            late = energy_stats(E[-GATE_STEPS:])
+            late["pass"] = late["max_dev"] <= GATE
         start, _ = e.cell_list(0)
         occ = np.diff(start)
         ktm = kernel_times(e, 20)
@@ -508,6 +539,7 @@ def coexistence(args, eng):
                      "pair_kernel": variant,
                      "energy_vs_abs_e0": energy_stats(E),
                      "energy_vs_ke0_plus_abs_pe0": energy_stats(E, scale=alt),
+                     "energy_last_10000_steps": late,
                      "E0": float(E[0]), "KE0": float(rec[0, 0, 1]), "PE0": float(rec[0, 0, 2]),
                      "load_imbalance": {"cell_occupancy_max_over_mean":
                                         float(occ.max() / occ.mean()),
@@ -543,6 +575,7 @@ def plain(args, eng):
             "steps": steps, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "plain FP64 N=864 (FCC 6^3) rho 0.8 T 1.0 rc 2.5 dt 0.005"},
             "energy_gate": gate,
+            "energy_timed_run": energy_stats(rec[:, 0, 3]),
             "log_every_10_first5": {"columns": ["time", "KE", "PE", "E", "W", "T", "MSD",
                                                 "P_virial"],
                                     "rows": [list(r) + [rho * (r[5] + r[4] / (3 * n))]

@@ -8,6 +8,8 @@
 #ifndef LJ_HOST_H
 #define LJ_HOST_H
 
+#include "lj_engine.h"   /* lj_thermo, status codes */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -101,6 +103,15 @@ int lj_simulation_run_batch(lj_simulation** sims, int nsim, int echo, int* statu
 double lj_simulation_potential(const lj_simulation* sim, double distance);
 double lj_simulation_virial(const lj_simulation* sim, double distance);
 double lj_simulation_force(const lj_simulation* sim, double distance);
+
+/* physics::ThermodynamicAnalyzer::result (physics/analyzers.cpp:44-159) over
+ * a tools::MovingSample window (tools/moving_sample.hpp:81-189) of the last
+ * sample_size of the count records given, oldest first -- the statistics an
+ * observation phase records: out[6] = temperature, density, total energy,
+ * pressure, specific heat, diffusion coefficient.  The same code the
+ * simulation's observation phase runs. */
+int lj_thermo_observation(const lj_thermo* records, int count, int sample_size, double density,
+                          int particle_count, double* out);
 
 /* Message of the most recent failing host-layer call on this thread. */
 const char* lj_host_last_error(void);

@@ -116,6 +116,23 @@ struct Observation {
     double temperature, density, total_energy, pressure, specific_heat, diffusion_coefficient;
 };
 
+// physics/analyzers.cpp:44-159: the observation of one window of samples --
+// means of T and E, P = rho (<T> + <W> / 3N) (:73-75), the heat capacity from
+// the temperature variance (Lebowitz-Percus-Verlet, :113-122) and D = slope of
+// MSD against time / 6 (:146-148).
+Observation analyze(const Sample& temps, const Sample& energies, const Sample& virials,
+                    const Sample2& msd, double density, int n)
+{
+    const double T = temps.mean();
+    const double E = energies.mean();
+    const double Wm = virials.mean();
+    const double P = density * (T + Wm / (3.0 * n));
+    const double rel = temps.variance(T) / (T * T);
+    const double cv = (3. / 2.) / (1 - (3. / 2.) * n * rel);
+    const double D = (1. / 6.) * msd.slope();
+    return {T, density, E, P, cv, D};
+}
+
 enum class Cmd { Advance, Observe, Adjust, Complete, Abort };
 
 struct Command {
@@ -208,17 +225,7 @@ struct ObservationP : Phase {
         virials.push(r.virial);
         msd.push({r.time, r.mean_square_displacement});
     }
-    Observation result() const
-    {
-        const double T = temps.mean();
-        const double E = energies.mean();
-        const double Wm = virials.mean();
-        const double P = density * (T + Wm / (3.0 * n));
-        const double rel = temps.variance(T) / (T * T);
-        const double cv = (3. / 2.) / (1 - (3. / 2.) * n * rel);
-        const double D = (1. / 6.) * msd.slope();
-        return {T, density, E, P, cv, D};
-    }
+    Observation result() const { return analyze(temps, energies, virials, msd, density, n); }
     void evaluate(std::deque<Command>& q, int t) override
     {
         if (t - last_obs >= interval) {
@@ -608,3 +615,24 @@ double lj_simulation_force(const lj_simulation* s, double d)
 }
 
 }  // extern "C"
+
+extern "C" int lj_thermo_observation(const lj_thermo* records, int count, int sample_size,
+                                     double density, int particle_count, double* out)
+{
+    if (!records || !out || count < 1 || sample_size < 1 || particle_count < 1)
+        return fail(LJ_E_INVALID, "lj_thermo_observation: bad arguments");
+    Sample temps(sample_size), energies(sample_size), virials(sample_size);
+    Sample2 msd(sample_size);
+    for (int k = 0; k < count; ++k) {
+        const lj_thermo& r = records[k];
+        temps.push(r.temperature);
+        energies.push(r.total_energy);
+        virials.push(r.virial);
+        msd.push({r.time, r.mean_square_displacement});
+    }
+    const Observation o = analyze(temps, energies, virials, msd, density, particle_count);
+    const double v[6] = {o.temperature, o.density, o.total_energy, o.pressure, o.specific_heat,
+                         o.diffusion_coefficient};
+    std::copy(v, v + 6, out);
+    return LJ_OK;
+}

@@ -167,8 +167,8 @@ struct lj_engine {
     bool prekicked = false;
     hipStream_t stream = nullptr;
     bool use_graphs = true;
-    hipGraphExec_t graph2 = nullptr;   // two steps starting from buffer set 0
-    hipGraphExec_t graphN = nullptr;   // kGraphSteps steps starting from buffer set 0
+    // graphs[s][l]: 2^(l+1) steps starting from buffer set s (2, 4, ..., 32)
+    hipGraphExec_t graphs[2][5] = {};
     int graph_cap = -1;               // record capacity baked into the graph
     const double* graph_rec = nullptr;   // record buffer baked into the graph
 
@@ -535,18 +535,19 @@ int ensure_records(lj_engine* e, int nsteps)
     return LJ_OK;
 }
 
-// Steps per long graph.  Each graph launch ends with a release of the caches
-// that costs several microseconds once the pair kernel has left the L2 full of
-// dirty lines, so long runs replay many steps per launch.  Any even count
-// returns the state to buffer set 0.
-constexpr int kGraphSteps = 32;
+// Step graphs of 2, 4, 8, 16 and 32 steps from either buffer set.  Each graph
+// launch ends with a release of the caches that costs several microseconds
+// once the pair kernel has left the L2 full of dirty lines, so a call replays
+// the fewest launches that add up to its step count (20 steps: 16 + 4).  An
+// even count returns the state to the buffer set it started from.
+constexpr int kGraphLevels = 5;
 
-// Captures nsteps steps starting from buffer set 0 into *out.
-int capture_steps(lj_engine* e, int nsteps, hipGraphExec_t* out)
+// Captures nsteps steps starting from buffer set `from` into *out.
+int capture_steps(lj_engine* e, int from, int nsteps, hipGraphExec_t* out)
 {
     const int saved = e->cur;
     const bool saved_sorted = e->sorted;
-    e->cur = 0;
+    e->cur = from;
     e->prekicked = false;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -571,20 +572,24 @@ int capture_steps(lj_engine* e, int nsteps, hipGraphExec_t* out)
 
 void drop_graphs(lj_engine* e)
 {
-    if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
-    if (e->graphN) (void)hipGraphExecDestroy(e->graphN);
-    e->graph2 = e->graphN = nullptr;
+    for (auto& set : e->graphs)
+        for (hipGraphExec_t& g : set) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 int build_graph(lj_engine* e)
 {
-    if (e->graph2 && e->graphN && e->graph_cap == e->rec_cap && e->graph_rec == e->d_records &&
-        !e->rec_buf)
+    if (e->graphs[1][kGraphLevels - 1] && e->graph_cap == e->rec_cap &&
+        e->graph_rec == e->d_records && !e->rec_buf)
         return LJ_OK;
     if (e->rec_buf) return fail(LJ_E_STATE, "step graphs are captured on the engine's own records only");
     drop_graphs(e);
-    int rc = capture_steps(e, 2, &e->graph2);
-    if (!rc) rc = capture_steps(e, kGraphSteps, &e->graphN);
+    int rc = LJ_OK;
+    for (int from = 0; from < 2 && !rc; ++from)
+        for (int l = 0; l < kGraphLevels && !rc; ++l)
+            rc = capture_steps(e, from, 2 << l, &e->graphs[from][l]);
     if (rc) {
         drop_graphs(e);
         return rc;
@@ -1073,20 +1078,14 @@ int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     int done = 0;
     if (graphs && nsteps >= 2) {
         if ((rc = build_graph(e))) return rc;
-        // the graph starts from buffer set 0: take one eager step if needed
-        if (e->cur != 0) {
-            if ((rc = enqueue_step(e))) return rc;
-            ++done;
-        }
-        while (nsteps - done >= kGraphSteps) {
-            HIP_TRY(hipGraphLaunch(e->graphN, e->stream));
-            done += kGraphSteps;
-            e->sorted = true;
-        }
-        while (nsteps - done >= 2) {
-            HIP_TRY(hipGraphLaunch(e->graph2, e->stream));
-            done += 2;
-            e->sorted = true;
+        // the largest graph that fits, repeatedly (an even count keeps e->cur)
+        for (int l = kGraphLevels - 1; l >= 0; --l) {
+            const int len = 2 << l;
+            while (nsteps - done >= len) {
+                HIP_TRY(hipGraphLaunch(e->graphs[e->cur][l], e->stream));
+                done += len;
+                e->sorted = true;
+            }
         }
     }
     while (done < nsteps) {

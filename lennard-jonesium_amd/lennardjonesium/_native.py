@@ -114,6 +114,8 @@ def host():
     lib.lj_sweep_sizes.argtypes = [ctypes.c_uint, c_i, c_i, c_i, P_i]
     lib.lj_format_double.restype = c_i
     lib.lj_format_double.argtypes = [c_d, ctypes.c_char_p, c_i]
+    lib.lj_thermo_observation.restype = c_i
+    lib.lj_thermo_observation.argtypes = [ctypes.POINTER(lj_thermo), c_i, c_i, c_d, c_i, P_d]
     _host = lib
     return lib
 

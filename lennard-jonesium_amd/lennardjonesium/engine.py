@@ -295,6 +295,21 @@ def sweep_sizes(seed, count, lo=5, hi=20):
     return out
 
 
+def thermo_observation(records, sample_size, density, particle_count):
+    """ThermodynamicAnalyzer::result over the last sample_size rows of
+    records ((k, 7): time, KE, PE, E, W, T, MSD) -> dict of the six
+    observation fields (csrc/host/simulation.cpp, the phase's own code)."""
+    rec = np.ascontiguousarray(np.asarray(records, dtype=np.float64).reshape(-1, 7))
+    out = np.zeros(6)
+    rc = _native.host().lj_thermo_observation(
+        rec.ctypes.data_as(ctypes.POINTER(_native.lj_thermo)), len(rec), int(sample_size),
+        float(density), int(particle_count), _dptr(out))
+    if rc < 0:
+        raise ValueError("invalid observation window")
+    return dict(zip(("temperature", "density", "total_energy", "pressure", "specific_heat",
+                     "diffusion_coefficient"), out.tolist()))
+
+
 def format_double(x):
     buf = ctypes.create_string_buffer(64)
     _native.host().lj_format_double(float(x), buf, 64)

@@ -44,17 +44,25 @@ def pair_terms(r, L, eps, sigma, cutoff):
     return F, 0.5 * e.sum(), 0.5 * w.sum()
 
 
-def trajectory_loss(eps, sigma, r0, v0, L, nsteps, dt, cutoff, density):
+def trajectory(eps, sigma, r0, v0, L, nsteps, dt, cutoff):
+    """Velocity Verlet from (r0, v0) with the forces F(r0) computed first;
+    yields (PE, W, T) after each step (positions are not wrapped: the 27
+    images cover every pair of a short trajectory)."""
     n = r0.shape[0]
     F, _, _ = pair_terms(r0, L, eps, sigma, cutoff)
     r, v = r0, v0
-    total = torch.zeros((), dtype=torch.float64)
     for _ in range(nsteps):
         vh = v + 0.5 * dt * F
         r = r + dt * vh
         F, pe, w = pair_terms(r, L, eps, sigma, cutoff)
         v = vh + 0.5 * dt * F
         ke = 0.5 * (v * v).sum()
-        T = (2.0 / 3.0) * ke / n
+        yield pe, w, (2.0 / 3.0) * ke / n
+
+
+def trajectory_loss(eps, sigma, r0, v0, L, nsteps, dt, cutoff, density):
+    n = r0.shape[0]
+    total = torch.zeros((), dtype=torch.float64)
+    for pe, w, T in trajectory(eps, sigma, r0, v0, L, nsteps, dt, cutoff):
         total = total + pe + density * (T + w / (3.0 * n))
     return total / nsteps

@@ -147,3 +147,67 @@ def test_bench_energy_stats_known_answer():
     assert got["max_dev"] == pytest.approx(1e-4, rel=1e-9)
     assert got["windowed"] == pytest.approx(5e-5, rel=1e-9)
     assert bench.energy_stats(E, scale=20.0)["max_dev"] == pytest.approx(5e-5, rel=1e-9)
+
+
+# ---- the observation statistics (physics/analyzers.cpp:44-159) -------------------
+
+def _records(T=None, tm=None, n=None):
+    """Records (k, 7) with the given temperatures and (time, MSD) pairs."""
+    k = len(T) if T is not None else len(tm)
+    rec = np.zeros((k, 7))
+    rec[:, 5] = T if T is not None else 1.0 + np.arange(k)
+    rec[:, 3] = -3.0
+    if tm is not None:
+        rec[:, 0] = [a for a, _ in tm]
+        rec[:, 6] = [b for _, b in tm]
+    else:
+        rec[:, 0] = np.arange(k)
+        rec[:, 6] = 2.0 * np.arange(k)
+    return rec
+
+
+@pytest.mark.parametrize("values,mean,var", [([2, 3], 2.5, 0.5), ([2, 3, 10], 5.0, 19.0),
+                                             ([2, 3, 8, 1], 4.0, 13.0)])
+def test_moving_sample_known_answers_scalar(values, mean, var):
+    """tests/cpp/lennardjonesium/tools/test_moving_sample.cpp:10-56 (a window
+    of 3 values; the fourth pushes the first out), read back through the
+    heat capacity of the host analyzer: Cv = 1.5 / (1 - 1.5 N var / <T>^2)."""
+    from lennardjonesium import engine as eng
+    n = 1
+    o = eng.thermo_observation(_records(T=values), 3, 1.0, n)
+    assert o["temperature"] == pytest.approx(mean, rel=1e-15)
+    var_back = (1.0 - 1.5 / o["specific_heat"]) * mean * mean / (1.5 * n)
+    assert var_back == pytest.approx(var, rel=1e-12)
+
+
+@pytest.mark.parametrize("pairs,slope", [([(1, 0), (0, 1)], -0.5 / 0.5),
+                                         ([(2, 0), (0, 3), (1, 0)], -1.5 / 1.0),
+                                         ([(2, 0), (0, 3), (1, 0), (5, 6)], 6.0 / 7.0)])
+def test_moving_sample_known_answers_covariance(pairs, slope):
+    """tests/cpp/lennardjonesium/tools/test_moving_sample.cpp:58-140 (vector
+    samples: covariance(0, 1) / covariance(0, 0) of the first two components
+    in a window of 3) as the diffusion coefficient D = slope / 6 of (time, MSD)."""
+    from lennardjonesium import engine as eng
+    o = eng.thermo_observation(_records(T=[1.0] * len(pairs), tm=pairs), 3, 1.0, 10)
+    assert o["diffusion_coefficient"] == pytest.approx(slope / 6.0, rel=1e-14)
+
+
+@pytest.mark.parametrize("k,size,seed", [(50, 50, 1), (120, 50, 2), (7, 5, 3)])
+def test_host_analyzer_equals_oracle(k, size, seed):
+    """The simulation's own analyzer against the oracle's restatement on
+    identical records: every observation field within 1e-12 relative."""
+    from lennardjonesium import engine as eng
+    from oracle import ljref
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((k, 7))
+    rec[:, 0] = 3.0 + 0.005 * np.arange(k)
+    rec[:, 5] = 0.8 + 0.01 * rng.standard_normal(k)
+    rec[:, 3] = -2500.0 + rng.standard_normal(k)
+    rec[:, 4] = -800.0 + 40 * rng.standard_normal(k)
+    rec[:, 6] = 0.05 * np.arange(k) + 0.01 * rng.random(k)
+    n, rho = 500, 0.8
+    got = eng.thermo_observation(rec, size, rho, n)
+    w = rec[-size:]
+    want = ljref.observation(w[:, 5], w[:, 3], w[:, 4], w[:, 0], w[:, 6], rho, n)
+    for key, v in want.items():
+        assert got[key] == pytest.approx(v, rel=1e-12, abs=0), key

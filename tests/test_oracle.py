@@ -322,3 +322,34 @@ def test_neighbor_counts_equal_pair_set(n, L, rc):
     idx, _ = ljref.pairs_naive(pos, L, rc)
     want = np.bincount(idx[:, 0], minlength=n) + np.bincount(idx[:, 1], minlength=n)
     assert np.array_equal(ljref.neighbor_counts(pos, L, rc), want)
+
+
+# ---- the torch restatement used as the independent reference gradient ----------
+
+@pytest.mark.parametrize("eps,sig", [(1.0, 1.0), (1.3, 0.95)])
+def test_torch_ref_forward_equals_ljref(eps, sig):
+    """oracle/torch_ref.py's trajectory (brute force over all 27 images, no
+    wrap, F(r0) before the first half kick) against ljref.run started with
+    the stored force F(r0): per-step PE, W and T, and the loss, within 1e-12
+    relative for 20 steps at N = 108."""
+    import torch
+    from oracle import torch_ref
+    st, L = ljref.initial_condition(108, 0.8, 1.0, seed=7)
+    L = L * sig
+    pos, vel = st.pos * sig, st.vel
+    s = ljref.State(pos=pos, vel=vel)
+    s.F, s.pe, s.vir = ljref.forces(pos, L, rc=2.5, eps=eps, sigma=sig)
+    ref = ljref.run(s, L, 0.005, 20, rc=2.5, eps=eps, sigma=sig)
+    with torch.no_grad():
+        got = [(pe.item(), w.item(), T.item()) for pe, w, T in torch_ref.trajectory(
+            torch.tensor(eps, dtype=torch.float64), torch.tensor(sig, dtype=torch.float64),
+            torch.tensor(pos), torch.tensor(vel), L, 20, 0.005, 2.5)]
+        loss = torch_ref.trajectory_loss(
+            torch.tensor(eps, dtype=torch.float64), torch.tensor(sig, dtype=torch.float64),
+            torch.tensor(pos), torch.tensor(vel), L, 20, 0.005, 2.5, 0.8).item()
+    got = np.array(got)
+    for col, k in ((0, 2), (1, 4), (2, 5)):
+        rel = np.abs(got[:, col] - ref[:, k]) / np.abs(ref[:, k])
+        assert rel.max() < 1e-12, (col, rel.max())
+    want = (ref[:, 2] + 0.8 * (ref[:, 5] + ref[:, 4] / (3 * 108))).mean()
+    assert abs(loss - want) < 1e-12 * abs(want)
