@@ -2,8 +2,9 @@
 
 Default workload ("liquid"): N=256,000 particles (FCC 40^3) at density 0.8442,
 T=0.72, cutoff 2.5, dt=0.005, seeded initial condition (seed 5489), in mixed
-precision (FP32 pair forces, FP64 positions / velocities / sums).  W warm-up
-steps from the initial condition, then K timed steps that continue from them,
+precision (FP32 pair forces, FP64 positions / velocities / sums).  The input
+is prepared by 1,000 untimed steps that melt the lattice into the liquid
+(--prepare), then W warm-up steps, then K timed steps that continue from them,
 each with its thermodynamic record.  The same in plain FP64 is timed beside
 it on the same GPU.  Independently of K, an energy-gate leg runs 1,000 + 10^4
 steps in each precision and reports |dE/E| over the 10^4 steps (the accuracy
@@ -217,13 +218,16 @@ def cpu_record(rate, seconds, what):
             "cpu_model": model, "host_cpu_count": ncpu}
 
 
-def timed_run(e, steps, warmup, sys_states):
-    """Loads the states, takes `warmup` steps, then times `steps` more steps
-    that continue from them (records included); buffers and step graphs are
-    prepared before the timed region."""
+def timed_run(e, steps, warmup, sys_states, prepare=0):
+    """Loads the states, advances them `prepare` steps (input preparation),
+    takes `warmup` steps, then times `steps` more steps that continue from
+    them (records included); buffers and step graphs are prepared before the
+    timed region."""
     for k, (p, v) in enumerate(sys_states):
         e.set_state(k, p, v)
-    e.reserve(max(steps, warmup))
+    e.reserve(max(steps, warmup, prepare))
+    if prepare:
+        e.step(prepare, records=False)
     if warmup:
         e.step(warmup)
     t0 = time.perf_counter()
@@ -247,7 +251,7 @@ def liquid(args, eng):
     log(f"workload N={n} L={L:.6f} steps={args.steps} warmup={args.warmup}")
     spec = [eng.SystemSpec(n, L)]
     em = eng.Engine(spec, precision="mixed", pair_kernel=args.pair_kernel)
-    recm, tm = timed_run(em, args.steps, args.warmup, [(pos, vel)])
+    recm, tm = timed_run(em, args.steps, args.warmup, [(pos, vel)], prepare=args.prepare)
     rate_m = n * args.steps / tm
     log(f"mixed: {tm:.4f} s, {rate_m:.4e} atom-steps/s")
     ktm = kernel_times(em)
@@ -256,7 +260,7 @@ def liquid(args, eng):
     res64, frms, kt64 = None, None, None
     if not args.no_fp64:
         e64 = eng.Engine(spec, precision="fp64", pair_kernel=args.pair_kernel)
-        rec64, t64 = timed_run(e64, args.steps, args.warmup, [(pos, vel)])
+        rec64, t64 = timed_run(e64, args.steps, args.warmup, [(pos, vel)], prepare=args.prepare)
         rate_64 = n * args.steps / t64
         log(f"fp64: {t64:.4f} s, {rate_64:.4e} atom-steps/s")
         kt64 = kernel_times(e64)
@@ -324,9 +328,12 @@ def liquid(args, eng):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_m,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pair forces + f64 state/sums",
-        "data": "synthetic (seeded FCC lattice + Maxwell-Boltzmann velocities, seed 5489)",
+        "data": (f"synthetic: seeded FCC lattice + Maxwell-Boltzmann velocities (seed 5489), "
+                 f"advanced {args.prepare} untimed NVE steps into the liquid (input preparation, "
+                 f"before the warm-up; the lattice melts within them) in each precision"),
         "config": {"workload": "NVE LJ liquid N=256,000 (FCC 40^3), rho=0.8442, T=0.72, "
                                "rc=2.5, dt=0.005, cell list rebuilt every step, mixed precision",
+                   "prepared_steps": args.prepare,
                    "particles": n, "steps": args.steps, "precision": "mixed",
                    "pair_kernel": args.pair_kernel, "parallelism": "single GPU"},
         "energy_gate": gates or None,
@@ -634,6 +641,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--prepare", type=int, default=1000,
+                    help="liquid: untimed steps that take the lattice into the liquid before "
+                         "the warm-up (0: time from the lattice)")
     ap.add_argument("--fp64-steps", type=int, default=2000,
                     help="coexistence: steps of the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true")

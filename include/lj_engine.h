@@ -247,13 +247,25 @@ int lj_engine_set_pair_split(lj_engine* e, double min_occupancy);
  * LJ_PAIR_PER_PARTICLE. */
 int lj_engine_pair_kernels(lj_engine* e, int* per_system);
 
-/* Project autotuner: times `steps` steps of each pair-kernel assignment --
- * all tiled, all per-particle, and per-system splits at cell occupancies 1.5,
- * 3, 6 and 12 that give distinct assignments -- on the engine's own systems
- * (from the current state, which is saved and restored, so tuning leaves no
- * trace), then keeps the fastest.  chosen / ms_per_step (optional) receive the
- * winning variant and its time per step; lj_engine_pair_kernels tells the
- * per-system choice. */
+/* Launch shape (no reference counterpart; neither knob changes any result):
+ * cells_per_block = home cells (one wave64 each) per workgroup of the mixed
+ * tiled pair kernel, 2, 4 (default) or 8; sort_wave_particles = particles the
+ * cell sort gives each wave (cells per wave = that / mean occupancy), 8..64
+ * (default 44). */
+int lj_engine_set_launch_shape(lj_engine* e, int cells_per_block, int sort_wave_particles);
+int lj_engine_launch_shape(lj_engine* e, int* cells_per_block, int* sort_wave_particles);
+
+/* Project autotuner, for the reference's one-Simulation-per-thread pool
+ * (api/simulation_pool.cpp:93-101) that this engine's batches replace.  It
+ * times `steps` steps of each pair-kernel assignment -- all tiled, all
+ * per-particle, and per-system splits at cell occupancies 1.5, 3, 6 and 12
+ * that give distinct assignments -- then, with the fastest assignment, each
+ * launch shape (cells per block 2 / 4 / 8 for mixed tiled systems, sort waves
+ * of 32 / 44 / 56 particles), on the engine's own systems from the current
+ * state, which is saved and restored, so tuning leaves no trace; it keeps the
+ * fastest.  chosen / ms_per_step (optional) receive the winning variant and
+ * its time per step; lj_engine_pair_kernels and lj_engine_launch_shape tell
+ * the choices. */
 int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step);
 
 /* Mean duration in milliseconds of each step-loop kernel over the last

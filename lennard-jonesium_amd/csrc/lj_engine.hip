@@ -114,6 +114,9 @@ struct lj_engine {
     int ncell_total = 0, ntile = 0;
     int ncblk = 0;                  // cell-space blocks (tiled pair kernel)
     int nsblk = 0;                  // sort blocks (groups of sort_k cells per wave)
+    int cap_cblk = 0, cap_sblk = 0;  // allocated sizes of the block maps (every shape fits)
+    int cells_per_block = ljk::kCellsPerBlockMixed;   // mixed tiled kernel: waves per workgroup
+    int sort_wave = 44;             // sort: target particles per wave
     int pair_variant = LJ_PAIR_CELL_TILED;
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
@@ -140,7 +143,7 @@ struct lj_engine {
     int *d_count = nullptr, *d_count_init = nullptr, *d_start = nullptr, *d_tile = nullptr;
     int* d_cell_xyz = nullptr;   // per cell: x | y << 10 | z << 20
     double* d_partial = nullptr;     // particle-space block sums
-    double* d_partial_c = nullptr;   // cell-space block sums (tiled pair kernels)
+    double* d_partial_c = nullptr;   // per-cell sums (tiled pair kernels)
     double *d_time = nullptr, *d_pe = nullptr, *d_vir = nullptr;
     int* d_step = nullptr;
     int* d_pending = nullptr;   // per system: a step awaits its finalize
@@ -243,11 +246,11 @@ struct lj_engine {
         f.cell_blocks = (mode == 0 ? sys[0].law != ljk::LAW_NONE : last_pair_force) ? 1 : 0;
         f.pending = d_pending;
         f.partial2 = d_partial2;
-        // first-level parts: about one block sum per thread, at most kMaxParts
+        // first-level part slots: the most parts any system uses (parts_of)
         int nbmax = 1;
         for (const SysDev& s : sys)
-            nbmax = std::max(nbmax, (f.cell_blocks && s.tiled) ? s.ncblk : s.nblk);
-        f.nf = std::min(ljk::kMaxParts, (nbmax + kBlock - 1) / kBlock);
+            nbmax = std::max(nbmax, (f.cell_blocks && s.tiled) ? s.ncell : s.nblk);
+        f.nf = std::min(ljk::kMaxParts, (nbmax + ljk::kPartSums - 1) / ljk::kPartSums);
         return f;
     }
 
@@ -400,9 +403,14 @@ template <int PREC, int LAW, int KICK>
 void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = p.cblk_list ? e->n_cblk_list : e->ncblk;
-    if constexpr (PREC == ljk::MIXED)
-        launch(e, ljk::k_pair_cells_mixed<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockMixed), p);
-    else
+    if constexpr (PREC == ljk::MIXED) {
+        if (e->cells_per_block == 2)
+            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 2>, dim3(grid), dim3(128), p);
+        else if (e->cells_per_block == 8)
+            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 8>, dim3(grid), dim3(512), p);
+        else
+            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 4>, dim3(grid), dim3(256), p);
+    } else
         launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockFp64), p);
 }
 
@@ -599,10 +607,60 @@ int build_graph(lj_engine* e)
     return LJ_OK;
 }
 
+// Cell-space blocks of the tiled pair kernel (cells_per_block cells per block
+// in mixed precision, one in FP64) and sort blocks (kBlock / 64 waves of
+// sort_k cells, about sort_wave particles per wave) of every system, into the
+// host copies; block maps of at most cap_cblk / cap_sblk entries.
+void layout_blocks(lj_engine* e, std::vector<int>& cblk_sys, std::vector<int>& sblk_sys)
+{
+    const int cpb = e->precision == LJ_PRECISION_MIXED ? e->cells_per_block
+                                                       : ljk::kCellsPerBlockFp64;
+    int cblk = 0, sblk = 0;
+    cblk_sys.clear();
+    sblk_sys.clear();
+    for (int k = 0; k < e->nsys; ++k) {
+        SysDev& s = e->sys[k];
+        s.cblk_first = cblk;
+        s.ncblk = std::max(1, (s.ncell + cpb - 1) / cpb);
+        cblk += s.ncblk;
+        cblk_sys.resize(cblk, k);
+        // a wave's cells rarely hold more than its 64 lanes
+        const double occ = s.ncell > 0 ? (double)s.n / s.ncell : 1.0;
+        s.sort_k = (int)std::min(63.0, std::max(1.0, std::floor((double)e->sort_wave / occ)));
+        s.sblk_first = sblk;
+        const int waves = std::max(1, (s.ncell + s.sort_k - 1) / s.sort_k);
+        sblk += (waves + kBlock / 64 - 1) / (kBlock / 64);
+        sblk_sys.resize(sblk, k);
+    }
+    e->ncblk = cblk;
+    e->nsblk = sblk;
+}
+
+// Uploads the block layout of the current launch shape (and the systems'
+// parameters that carry it); the step graphs are rebuilt on their next use.
+int upload_layout(lj_engine* e)
+{
+    std::vector<int> cblk_sys, sblk_sys;
+    layout_blocks(e, cblk_sys, sblk_sys);
+    if (e->ncblk > e->cap_cblk || e->nsblk > e->cap_sblk)
+        return fail(LJ_E_INVALID, "launch shape needs more blocks than were allocated");
+    drop_graphs(e);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(e->d_sys, e->sys.data(), e->nsys * sizeof(SysDev), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_cblk_sys, cblk_sys.data(), cblk_sys.size() * sizeof(int),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_sblk_sys, sblk_sys.data(), sblk_sys.size() * sizeof(int),
+                      hipMemcpyHostToDevice));
+    return LJ_OK;
+}
+
+int assign_pair_kernels(lj_engine* e, bool force_lists = false);
+
 // Per-system pair kernel from the engine's policy: all tiled, all per-particle,
 // or tiled where the mean cell occupancy N / cells reaches split_occupancy.
-// Uploads the flags; the step graphs are rebuilt on their next use.
-int assign_pair_kernels(lj_engine* e)
+// Uploads the flags and block lists (also when force_lists is set, after the
+// block layout changed); the step graphs are rebuilt on their next use.
+int assign_pair_kernels(lj_engine* e, bool force_lists)
 {
     int nt = 0;
     bool changed = false;
@@ -616,7 +674,7 @@ int assign_pair_kernels(lj_engine* e)
         nt += t;
     }
     e->n_tiled = nt;
-    if (changed) {
+    if (changed || force_lists) {
         drop_graphs(e);
         std::vector<int> cl, bl;
         for (const SysDev& s : e->sys) {
@@ -779,18 +837,10 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         s.atom_off = atom_off;
         s.blk_first = blk;
         s.nblk = slots / kBlock;
-        s.cblk_first = cblk;
-        const int cpb = precision == LJ_PRECISION_MIXED ? ljk::kCellsPerBlockMixed
-                                                        : ljk::kCellsPerBlockFp64;
-        s.ncblk = std::max(1, (s.ncell + cpb - 1) / cpb);
-        cblk += s.ncblk;
-        // sort: about 44 particles per wave (64 lanes), so that a wave's
-        // cells rarely hold more than 64
-        const double occ = s.ncell > 0 ? (double)s.n / s.ncell : 1.0;
-        s.sort_k = (int)std::min(63.0, std::max(1.0, std::floor(44.0 / occ)));
-        s.sblk_first = sblk;
-        const int waves = std::max(1, (s.ncell + s.sort_k - 1) / s.sort_k);
-        sblk += (waves + kBlock / 64 - 1) / (kBlock / 64);
+        // block maps sized for every launch shape: one cell per tiled block,
+        // one cell per sort wave
+        cblk += std::max(1, s.ncell);
+        sblk += (std::max(1, s.ncell) + kBlock / 64 - 1) / (kBlock / 64);
         if (e->has_cells) {
             s.cell_off = cell_off;
             count_init.resize(cell_off + s.ncell + 1, 0);
@@ -804,18 +854,15 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     e->nslot = atom_off;
     for (const SysDev& s : e->sys) e->n_tiled += s.tiled;
     e->nblk = blk;
-    e->ncblk = cblk;
-    e->nsblk = sblk;
+    e->cap_cblk = cblk;
+    e->cap_sblk = sblk;
     e->ncell_total = cell_off;
     e->ntile = std::max(1, (cell_off + kScanTile - 1) / kScanTile);
 
-    std::vector<int> blk_sys(blk), cblk_sys(cblk), sblk_sys(sblk);
-    for (int k = 0; k < nsys; ++k) {
+    std::vector<int> blk_sys(blk), cblk_sys, sblk_sys;
+    for (int k = 0; k < nsys; ++k)
         for (int b = 0; b < e->sys[k].nblk; ++b) blk_sys[e->sys[k].blk_first + b] = k;
-        for (int b = 0; b < e->sys[k].ncblk; ++b) cblk_sys[e->sys[k].cblk_first + b] = k;
-        const int last = (k + 1 < nsys) ? e->sys[k + 1].sblk_first : sblk;
-        for (int b = e->sys[k].sblk_first; b < last; ++b) sblk_sys[b] = k;
-    }
+    layout_blocks(e, cblk_sys, sblk_sys);
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -839,7 +886,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_sblk_sys, sblk)) ||
         (rc = dev_alloc(&e->d_force, 3 * ns)) ||
         (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
-        (rc = dev_alloc(&e->d_partial_c, 4 * (size_t)cblk)) ||
+        (rc = dev_alloc(&e->d_partial_c, 4 * (size_t)std::max(1, cell_off))) ||
         (rc = dev_alloc(&e->d_cblk_list, (size_t)cblk)) || (rc = dev_alloc(&e->d_blk_list, (size_t)blk)) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
@@ -882,8 +929,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
             ident[e->sys[k].atom_off + i] = i < e->sys[k].n ? i : 0;
     bool ok = h2d(e->d_sys, e->sys.data(), nsys * sizeof(SysDev)) &&
               h2d(e->d_blk_sys, blk_sys.data(), blk * sizeof(int)) &&
-              h2d(e->d_cblk_sys, cblk_sys.data(), cblk * sizeof(int)) &&
-              h2d(e->d_sblk_sys, sblk_sys.data(), sblk * sizeof(int)) &&
+              h2d(e->d_cblk_sys, cblk_sys.data(), cblk_sys.size() * sizeof(int)) &&
+              h2d(e->d_sblk_sys, sblk_sys.data(), sblk_sys.size() * sizeof(int)) &&
               h2d(e->d_force, zeros.data(), 3 * ns * sizeof(double)) &&
               h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
@@ -1071,10 +1118,10 @@ int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     int rc;
     const ljk::FinalArgs f = e->fargs(0);
     const long long nrec = (long long)nsteps * e->nsys * 7;
-    hipLaunchKernelGGL(ljk::k_zero, dim3((unsigned)std::min<long long>(4096, (nrec + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, e->stream, f.records, nrec);
-    hipLaunchKernelGGL(ljk::k_reset_steps, dim3((e->nsys + 255) / 256), dim3(256), 0, e->stream,
-                       e->d_step, e->nsys);
+    const long long nthr = std::max<long long>(nrec, e->nsys);
+    hipLaunchKernelGGL(ljk::k_begin_call,
+                       dim3((unsigned)std::min<long long>(4096, (nthr + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, e->stream, f.records, nrec, e->d_step, e->nsys);
     int done = 0;
     if (graphs && nsteps >= 2) {
         if ((rc = build_graph(e))) return rc;
@@ -1358,6 +1405,30 @@ int lj_engine_set_pair_split(lj_engine* e, double min_occupancy)
     return assign_pair_kernels(e);
 }
 
+int lj_engine_set_launch_shape(lj_engine* e, int cells_per_block, int sort_wave_particles)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (cells_per_block != 2 && cells_per_block != 4 && cells_per_block != 8)
+        return fail(LJ_E_INVALID, "cells_per_block must be 2, 4 or 8");
+    if (sort_wave_particles < 8 || sort_wave_particles > 64)
+        return fail(LJ_E_INVALID, "sort_wave_particles must be in [8, 64]");
+    if (cells_per_block == e->cells_per_block && sort_wave_particles == e->sort_wave) return LJ_OK;
+    e->cells_per_block = cells_per_block;
+    e->sort_wave = sort_wave_particles;
+    if ((rc = upload_layout(e))) return rc;
+    return assign_pair_kernels(e, true);
+}
+
+int lj_engine_launch_shape(lj_engine* e, int* cells_per_block, int* sort_wave_particles)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (cells_per_block) *cells_per_block = e->cells_per_block;
+    if (sort_wave_particles) *sort_wave_particles = e->sort_wave;
+    return LJ_OK;
+}
+
 int lj_engine_pair_kernels(lj_engine* e, int* per_system)
 {
     int rc = check_engine(e);
@@ -1425,6 +1496,7 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     };
     // every exit after the save goes through here: the state and the kernel
     // policy the engine had on entry are put back unless a winner was chosen
+    const int cpb_entry = e->cells_per_block, sw_entry = e->sort_wave;
     auto finish = [&](int code, bool restored) {
         if (!restored) (void)restore();
         (void)hipFree(sid);
@@ -1432,6 +1504,7 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
             const std::string msg = g_err;
             e->split_occupancy = occ0;
             e->pair_variant = variant0;
+            (void)lj_engine_set_launch_shape(e, cpb_entry, sw_entry);
             (void)assign_pair_kernels(e);
             g_err = msg;
         }
@@ -1476,6 +1549,34 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     }
     e->split_occupancy = best_c.variant == LJ_PAIR_BY_OCCUPANCY ? best_c.occupancy : occ0;
     if ((rc = lj_engine_set_pair_kernel(e, best_c.variant))) return finish(rc, true);
+    // then the launch shape with that assignment: waves per workgroup of the
+    // mixed tiled kernel and particles per sort wave (neither changes any
+    // result: the tiled kernels keep one record of sums per cell, and the
+    // sort's order does not depend on how cells are grouped into waves)
+    const int cpb0 = e->cells_per_block, sw0 = e->sort_wave;
+    int best_cpb = cpb0, best_sw = sw0;
+    std::vector<std::pair<int, int>> shapes;
+    const bool tiled_mixed = e->precision == LJ_PRECISION_MIXED && e->n_tiled > 0;
+    for (int cpb : {2, 4, 8})
+        if (tiled_mixed || cpb == cpb0)
+            for (int sw : {32, 44, 56}) shapes.push_back({cpb, sw});
+    for (const auto& sh : shapes) {
+        if (sh.first == cpb0 && sh.second == sw0) continue;   // timed above
+        if ((rc = lj_engine_set_launch_shape(e, sh.first, sh.second))) return finish(rc, true);
+        if ((rc = lj_engine_reserve(e, steps))) return finish(rc, false);
+        if ((rc = lj_engine_step(e, 2, nullptr))) return finish(rc, false);
+        hipError_t h = hipEventRecord(e->ev0, e->stream);
+        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event record failed"), false);
+        if ((rc = lj_engine_step(e, steps, nullptr))) return finish(rc, false);
+        float ms = 0;
+        h = hipEventRecord(e->ev1, e->stream);
+        if (h == hipSuccess) h = hipEventSynchronize(e->ev1);
+        if (h == hipSuccess) h = hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event timing failed"), false);
+        if (ms < best) { best = ms; best_cpb = sh.first; best_sw = sh.second; }
+        if (restore() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: restore failed"), true);
+    }
+    if ((rc = lj_engine_set_launch_shape(e, best_cpb, best_sw))) return finish(rc, true);
     (void)finish(LJ_OK, true);
     if (chosen) *chosen = best_c.variant;
     if (ms_per_step) *ms_per_step = best / steps;

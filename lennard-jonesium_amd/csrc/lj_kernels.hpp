@@ -37,10 +37,11 @@ namespace ljk {
 
 constexpr int kBlock = 256;        // slots per block in particle-space kernels
 constexpr int kScanTile = 1024;    // cells per block in the scan kernels
-constexpr int kCellsPerBlockMixed = 4;  // home cells (one wave64 each) per block, mixed tiled kernel
+constexpr int kCellsPerBlockMixed = 4;  // default home cells (one wave64 each) per block, mixed tiled kernel
 constexpr int kCellsPerBlockFp64 = 1;   // FP64 tiled kernel: single-wave blocks retire independently
 constexpr int kHistBins = 1024;    // LDS histogram window of the cell count (cells)
 constexpr int kMaxParts = 64;      // first-level parts per system of the record sums
+constexpr int kPartSums = 1024;    // partial sums per first-level part (below kMaxParts parts)
 
 enum Law { LAW_LJ = 0, LAW_CONSTANT = 1, LAW_NONE = 2 };
 enum Prec { FP64 = 0, MIXED = 1 };
@@ -56,7 +57,7 @@ struct SysDev {
     int periodic;
     int blk_first;     // first particle-space block
     int nblk;          // blocks of this system
-    int cblk_first;    // first cell-space block (kCellsPerBlockMixed / kCellsPerBlockFp64 cells each)
+    int cblk_first;    // first cell-space block (the tiled kernel's cells per block each)
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
     int tiled;         // pair forces from the tiled kernel (1) or the per-particle kernel (0)
@@ -221,7 +222,7 @@ struct StepArgs {
     int* scell;                         // cell of each sorted slot
     float4* lpos;                       // mixed: cell-local float coordinates
     double* partial;                    // per particle-space block: sum v^2, PE, W, sum d^2
-    double* partial_c;                  // the same per cell-space block (tiled pair kernels)
+    double* partial_c;                  // the same per cell (tiled pair kernels), indexed like cells
     int skip_tiled;                     // per-particle pair kernel: leave tiled systems alone
     const int* cblk_list;               // tiled pair kernels: cell-space blocks to run (null: all)
     const int* blk_list;                // per-particle pair kernel: blocks to run (null: all)
@@ -239,7 +240,7 @@ struct StepArgs {
 struct FinalArgs {
     const SysDev* sys;
     const double* partial;    // particle-space block sums
-    const double* partial_c;  // cell-space block sums (systems with tiled pair forces)
+    const double* partial_c;  // per-cell sums (systems with tiled pair forces)
     double* time;      // per system
     double* pe;        // per system (last force calculation)
     double* vir;
@@ -1133,10 +1134,9 @@ struct SliceGeomTable {
 __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 
 
-template <int LAW, int KICK>
-__global__ __launch_bounds__(64 * kCellsPerBlockMixed, 6) void k_pair_cells_mixed(StepArgs p)
+template <int LAW, int KICK, int NW>
+__global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
 {
-    constexpr int NW = kCellsPerBlockMixed;
     // One record per wave, so that every per-lane LDS address is one base
     // register plus an immediate offset.
     struct alignas(16) WaveLds {
@@ -1156,10 +1156,11 @@ __global__ __launch_bounds__(64 * kCellsPerBlockMixed, 6) void k_pair_cells_mixe
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveLds& W = wl[wid];
-    const int lc = (b - S.cblk_first) * kCellsPerBlockMixed + wid;
+    const int lc = (b - S.cblk_first) * NW + wid;
+    if (lc >= S.ncell) return;   // past the system's last cell (no barrier follows)
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
-    if (lc < S.ncell && (!KICK || S.active)) {
+    if (!KICK || S.active) {
         const int c = S.cell_off + lc;
         int cx, cy, cz;
         cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
@@ -1424,12 +1425,15 @@ __global__ __launch_bounds__(64 * kCellsPerBlockMixed, 6) void k_pair_cells_mixe
             }
         }
     }
+    // this cell's sums (one record per cell, so the block shape does not
+    // change any result)
     double out[4];
-    block_sum4_lds<NW>(wacc, reinterpret_cast<double*>(W.st), out);
-    if (threadIdx.x == 0) {
+    block_sum4_lds<1>(wacc, reinterpret_cast<double*>(W.st), out);
+    if (lane == 0) {
+        const int c = S.cell_off + lc;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
-        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
+        for (int q = 0; q < 4; ++q) p.partial_c[4 * (size_t)c + q] = out[q];
+        if (KICK && lc == 0) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
 
@@ -1489,10 +1493,11 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lc = (b - S.cblk_first) * kCellsPerBlockFp64 + wid;
+    if (lc >= S.ncell) return;   // past the system's last cell (no barrier follows)
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
     double* sdw = sd[wid];
 
-    if (lc < S.ncell && (!KICK || S.active)) {
+    if (!KICK || S.active) {
         const int c = S.cell_off + lc;
         int cx, cy, cz;
         cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
@@ -1771,11 +1776,12 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
         }
     }
     double out[4];
-    block_sum4_lds<NW>(wacc, sdw, out);
-    if (threadIdx.x == 0) {
+    block_sum4_lds<1>(wacc, sdw, out);
+    if (lane == 0) {
+        const int c = S.cell_off + lc;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p.partial_c[4 * b + q] = out[q];
-        if (KICK && b == S.cblk_first) p.pending[k] = 1;   // the step awaits its finalize
+        for (int q = 0; q < 4; ++q) p.partial_c[4 * (size_t)c + q] = out[q];
+        if (KICK && lc == 0) p.pending[k] = 1;   // the step awaits its finalize
     }
 }
 
@@ -1844,18 +1850,18 @@ __device__ void finalize_range(const FinalArgs& f, int k, const double* part, in
 __device__ __forceinline__ const double* block_sums(const FinalArgs& f, const SysDev& S, int& nb)
 {
     const bool cells = f.cell_blocks && S.tiled;
-    nb = cells ? S.ncblk : S.nblk;
-    return cells ? f.partial_c + 4 * (size_t)S.cblk_first : f.partial + 4 * (size_t)S.blk_first;
+    nb = cells ? S.ncell : S.nblk;
+    return cells ? f.partial_c + 4 * (size_t)S.cell_off : f.partial + 4 * (size_t)S.blk_first;
 }
 
-// First-level parts of system k's record sums: about one block sum per
-// thread, from system k's own block count only, so that a system's records
-// do not depend on the other systems of its engine.
+// First-level parts of system k's record sums: one per kPartSums partial
+// sums (four per thread of a part's block), from system k's own count only,
+// so that a system's records do not depend on the other systems of its engine.
 __device__ int parts_of(const FinalArgs& f, int k)
 {
     int nb;
     (void)block_sums(f, f.sys[k], nb);
-    return max(1, min(kMaxParts, (nb + kBlock - 1) / kBlock));
+    return max(1, min(kMaxParts, (nb + kPartSums - 1) / kPartSums));
 }
 
 // First level: part r of system k's parts_of(f, k) contiguous parts of its
@@ -1897,19 +1903,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize_reduced(FinalArgs f)
     finalize_range<kBlock>(f, k, f.partial2 + 4 * (size_t)k * f.nf, parts_of(f, k));
 }
 
-__global__ void k_reset_steps(int* step, int nsys)
+// Start of a step call: clears the n doubles of its record rows (rows of
+// frozen systems are not written by the steps and read back as zeros) and
+// resets every system's record counter.
+__global__ __launch_bounds__(kBlock) void k_begin_call(double* v, long long n, int* step, int nsys)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nsys) step[k] = 0;
-}
-
-// Clears n doubles (the record rows of a call: rows of frozen systems are
-// not written by the steps and read back as zeros).
-__global__ __launch_bounds__(kBlock) void k_zero(double* v, long long n)
-{
-    for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
-         k += (long long)gridDim.x * kBlock)
-        v[k] = 0.0;
+    const long long t0 = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long k = t0; k < n; k += stride) v[k] = 0.0;
+    for (long long k = t0; k < nsys; k += stride) step[k] = 0;
 }
 
 // v *= scale for the slots of one system

@@ -89,6 +89,8 @@ def engine():
         "lj_engine_set_pair_split": (c_i, [V, c_d]),
         "lj_engine_pair_kernels": (c_i, [V, P_i]),
         "lj_engine_autotune": (c_i, [V, c_i, P_i, P_d]),
+        "lj_engine_set_launch_shape": (c_i, [V, c_i, c_i]),
+        "lj_engine_launch_shape": (c_i, [V, P_i, P_i]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),
         "lj_engine_kernel_times": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_char_p), P_d, P_i]),
     }

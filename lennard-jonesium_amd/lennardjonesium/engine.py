@@ -238,6 +238,17 @@ class Engine:
         names = {k: n for n, k in PAIR_KERNELS.items()}
         return [names[int(v)] for v in out]
 
+    def set_launch_shape(self, cells_per_block=4, sort_wave_particles=44):
+        """Waves per workgroup of the mixed tiled kernel (2, 4, 8) and particles
+        per sort wave (8..64); results do not depend on either."""
+        check(self.lib.lj_engine_set_launch_shape(self.h, int(cells_per_block),
+                                                  int(sort_wave_particles)))
+
+    def launch_shape(self):
+        cpb, sw = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.lj_engine_launch_shape(self.h, ctypes.byref(cpb), ctypes.byref(sw)))
+        return cpb.value, sw.value
+
     def autotune(self, steps=20):
         """Times each pair-kernel variant on this workload and keeps the fastest;
         returns (variant name, ms per step)."""

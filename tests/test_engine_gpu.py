@@ -444,6 +444,35 @@ def test_autotune_leaves_no_trace():
     assert np.array_equal(ref.step(30), tuned.step(30))
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_launch_shapes_give_identical_records(precision):
+    """Waves per workgroup of the mixed tiled kernel and particles per sort
+    wave change how work is grouped, not what is computed: every shape gives
+    the default shape's records and state bit for bit (a dense and a dilute
+    system in one engine, both pair kernels)."""
+    eng = _engine_mod()
+    specs, states = [], []
+    for k, (n, rho) in enumerate([(13500, 0.8), (4000, 0.1)]):
+        pos, vel, L = eng.initial_condition(n, rho, 1.0, seed=20 + k)
+        specs.append(eng.SystemSpec(n, L))
+        states.append((pos, vel))
+    ref = None
+    for cpb in (4, 2, 8):
+        for sw in (44, 32, 56):
+            with eng.Engine(specs, precision=precision, pair_kernel="split") as e:
+                e.set_launch_shape(cpb, sw)
+                assert e.launch_shape() == (cpb, sw)
+                for k, st in enumerate(states):
+                    e.set_state(k, *st)
+                rec = e.step(25)
+                fin = e.get_state(0)["vel"]
+            if ref is None:
+                ref = (rec, fin)
+            else:
+                assert np.array_equal(rec, ref[0]), (cpb, sw)
+                assert np.array_equal(fin, ref[1]), (cpb, sw)
+
+
 def test_autotune_after_steps_leaves_valid_cell_lists():
     """Tuning after steps (the slots already sorted) must not leave the cell
     ranges of a trial's last sort behind: the pair set and g(r) read
