@@ -428,9 +428,11 @@ def sweep(args, eng):
             e.set_state(k, p, v)
         variant, ms = e.autotune(10)
         kinds = e.pair_kernels()
+        shape = e.launch_shape()
         n_tiled = sum(1 for x in kinds if x == "tiled")
         log(f"{prec}: autotuner chose {variant} ({ms:.3f} ms/step; "
-            f"{n_tiled} replicas tiled, {len(kinds) - n_tiled} per-particle)")
+            f"{n_tiled} replicas tiled, {len(kinds) - n_tiled} per-particle; "
+            f"{shape[0]} cells per tiled block, sort waves of {shape[1]} particles)")
         rec, counts, ns, t = sweep_leg(e, states, steps, args.warmup, len(specs))
         e.close()
         rate = total * steps / t
@@ -444,6 +446,8 @@ def sweep(args, eng):
         work_ok = sum(meta[k][2] for k in range(len(specs)) if ok[k]) * steps
         worst = int(np.argmax(drifts))
         out[prec] = dict(value=rate, seconds=t, pair_kernel=variant,
+                         launch_shape={"cells_per_tiled_block": shape[0],
+                                       "sort_wave_particles": shape[1]},
                          replicas_tiled=n_tiled, replicas_per_particle=len(kinds) - n_tiled,
                          energy_definition="max_t |E_t - E_0| / max(|E_0|, KE_0 + |PE_0|) "
                                            "over the run, per replica",
@@ -529,6 +533,7 @@ def coexistence(args, eng):
         # from the lattice)
         e.step(1000, records=False)
         variant, _ = e.autotune(20)
+        shape = e.launch_shape()
         nst = steps if prec == "mixed" else min(steps, args.fp64_steps)
         rec, t = timed_run(e, nst, 0, [(pos, vel)])
         E = rec[:, 0, 3]
@@ -544,6 +549,8 @@ def coexistence(args, eng):
         e.close()
         out[prec] = {"value": n * nst / t, "steps": nst, "ms_per_step": 1e3 * t / nst,
                      "pair_kernel": variant,
+                     "launch_shape": {"cells_per_tiled_block": shape[0],
+                                      "sort_wave_particles": shape[1]},
                      "energy_vs_abs_e0": energy_stats(E),
                      "energy_vs_ke0_plus_abs_pe0": energy_stats(E, scale=alt),
                      "energy_last_10000_steps": late,
