@@ -452,6 +452,9 @@ def sweep(args, eng):
                          energy_definition="max_t |E_t - E_0| / max(|E_0|, KE_0 + |PE_0|) "
                                            "over the run, per replica",
                          replicas_over_gate=int((~ok).sum()),
+                         over_gate=[{"k": int(k), "T": meta[k][0], "rho": meta[k][1],
+                                     "N": meta[k][2], "dev": float(drifts[k])}
+                                    for k in np.argsort(-drifts) if not ok[k]],
                          value_replicas_within_gate=work_ok / t,
                          max_replica_dev=float(drifts.max()),
                          median_replica_dev=float(np.median(drifts)),
