@@ -240,6 +240,43 @@ def test_initial_condition_invariants(n, T):
     assert np.all(st.F == 0)
 
 
+# Four particles at (0,0,0), (0,0,2), (2,2,0), (2,2,2): centre of mass (1,1,1)
+# and the two velocity sets of test_measurements.cpp:80-188 (rotation about z,
+# then about x) with their known kinetic energy 4, temperature 2/3 and angular
+# momenta (0,0,8) and (-8,4,0) about the centre.
+_FOUR = np.array([[0, 0, 0], [0, 0, 2], [2, 2, 0], [2, 2, 2]], dtype=np.float64)
+
+
+@pytest.mark.parametrize("vel,ang", [
+    ([[1, -1, 0], [1, -1, 0], [-1, 1, 0], [-1, 1, 0]], [0, 0, 8]),
+    ([[0, -1, 1], [0, 1, 1], [0, -1, -1], [0, 1, -1]], [-8, 4, 0]),
+])
+def test_four_particle_measurements_and_zeroing(vel, ang):
+    vel = np.array(vel, dtype=np.float64)
+    st = ljref.State(pos=_FOUR.copy(), vel=vel.copy())
+    rec = ljref.measure(st)                       # time, KE, PE, E, W, T, MSD
+    assert rec[1] == 4.0 and rec[5] == pytest.approx(2.0 / 3.0, rel=1e-15)
+    np.testing.assert_array_equal(np.cross(_FOUR - 1.0, vel).sum(0), ang)
+    # physics/transformations.cpp:35-92: zeroing P and L about the centre
+    v = vel.copy()
+    ljref.lib().ljref_zero_momenta(4, ljref._d(np.ascontiguousarray(_FOUR)), ljref._d(v))
+    assert np.abs(v.sum(0)).max() < 1e-14
+    assert np.abs(np.cross(_FOUR - 1.0, v).sum(0)).max() < 1e-14
+
+
+def test_zeroing_removes_a_rigid_motion_exactly():
+    """v = V + w x (r - com) is a rigid translation plus rotation: removing the
+    momentum and then the angular momentum (I dw = -L) must leave v = 0.  A
+    wrong sign or a transposed cross product would double it instead."""
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 5, (50, 3))
+    com = pos.mean(0)
+    V, w = np.array([0.3, -1.1, 0.7]), np.array([0.9, 0.2, -0.5])
+    v = V + np.cross(w, pos - com)
+    ljref.lib().ljref_zero_momenta(50, ljref._d(np.ascontiguousarray(pos)), ljref._d(v))
+    assert np.abs(v).max() < 1e-13
+
+
 # ---- end-to-end fixture: notebooks/quick_single_run/data (reference binary output)
 
 def _csv(path):
