@@ -807,20 +807,43 @@ __device__ __forceinline__ void for_each_neighbour_fp64(const SysDev& S, const S
             int nby, iy;
             nb_axis(cy, oy, S.nc[1], nby, iy);
             const double shy = iy * S.L[1];
-            for (int oz = -1; oz <= 1; ++oz) {
-                int nbz, iz;
-                nb_axis(cz, oz, S.nc[2], nbz, iz);
-                const double shz = iz * S.L[2];
+            // the row's three cells (oz = -1, 0, 1) walked as one run of nt
+            // candidates in the same order as three separate loops, each
+            // position loaded one candidate ahead
+            int jb[3], je[3], izs[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                int nbz;
+                nb_axis(cz, k - 1, S.nc[2], nbz, izs[k]);
                 const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
-                const int j0 = p.start[nb], j1 = p.start[nb + 1];
-                for (int j = j0; j < j1; ++j) {
-                    if (j == s) continue;
-                    const double dx = (xi - p.pos.x[j]) - shx;
-                    const double dy = (yi - p.pos.y[j]) - shy;
-                    const double dz = (zi - p.pos.z[j]) - shz;
+                jb[k] = p.start[nb];
+                je[k] = p.start[nb + 1];
+            }
+            const int n0 = je[0] - jb[0];
+            const int n01 = n0 + (je[1] - jb[1]);
+            const int nt = n01 + (je[2] - jb[2]);
+            auto cand = [&](int t) {
+                return t < n0 ? jb[0] + t : (t < n01 ? jb[1] + (t - n0) : jb[2] + (t - n01));
+            };
+            const int last = p.nslot - 1;
+            int j = cand(0);
+            int jc = min(j, last);
+            double xj = p.pos.x[jc], yj = p.pos.y[jc], zj = p.pos.z[jc];
+            for (int t = 0; t < nt; ++t) {
+                const int jn = cand(t + 1);
+                const int jnc = min(jn, last);
+                const double xn = p.pos.x[jnc], yn = p.pos.y[jnc], zn = p.pos.z[jnc];
+                const int iz = t < n0 ? izs[0] : (t < n01 ? izs[1] : izs[2]);
+                if (j != s) {
+                    const double shz = iz * S.L[2];
+                    const double dx = (xi - xj) - shx;
+                    const double dy = (yi - yj) - shy;
+                    const double dz = (zi - zj) - shz;
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (r2 < rc2) visit(j, dx, dy, dz, r2, ix, iy, iz);
                 }
+                j = jn;
+                xj = xn; yj = yn; zj = zn;
             }
         }
     }
@@ -967,9 +990,13 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
                     int nby, iy;
                     nb_axis(cy, oy, S.nc[1], nby, iy);
                     const float shy = S.shf[1][3 * (oy + 1) + iy + 1];
-                    // the row's three cells: their ranges, then their first
-                    // members, each fetched in one round (dilute systems hold
-                    // a particle or two per cell)
+                    // the row's three cells, walked as one run of nt
+                    // candidates (cell 0's members, then cell 1's, then cell
+                    // 2's: the order of three separate loops), so a wave
+                    // waits for the longest row of its lanes rather than for
+                    // the longest cell of each of the three; every position
+                    // is loaded one candidate ahead (dilute systems hold a
+                    // particle or two per cell)
                     int jb[3], je[3];
                     float shz[3];
 #pragma unroll
@@ -981,13 +1008,21 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
                         jb[k] = p.start[nb];
                         je[k] = p.start[nb + 1];
                     }
-                    float4 l0[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) l0[k] = p.lpos[min(jb[k], last)];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        if (jb[k] < je[k]) visit(jb[k], l0[k], shx, shy, shz[k]);
-                        for (int j = jb[k] + 1; j < je[k]; ++j) visit(j, p.lpos[j], shx, shy, shz[k]);
+                    const int n0 = je[0] - jb[0];
+                    const int n01 = n0 + (je[1] - jb[1]);
+                    const int nt = n01 + (je[2] - jb[2]);
+                    auto cand = [&](int t) {   // slot of the row's t-th candidate
+                        return t < n0 ? jb[0] + t : (t < n01 ? jb[1] + (t - n0) : jb[2] + (t - n01));
+                    };
+                    int j = cand(0);
+                    float4 lj = p.lpos[min(j, last)];
+                    for (int t = 0; t < nt; ++t) {
+                        const int jn = cand(t + 1);
+                        const float4 ln = p.lpos[min(jn, last)];
+                        const float sz = t < n0 ? shz[0] : (t < n01 ? shz[1] : shz[2]);
+                        visit(j, lj, shx, shy, sz);
+                        j = jn;
+                        lj = ln;
                     }
                 }
             }
