@@ -1400,10 +1400,13 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
                     // the force is accumulated as (F/r) / 24 = (g / 24) / r^2 + cw / 24
                     // and scaled by 24 in double once per particle
                     float gs = 0.f, s6s = 0.f, r2s = 0.f;
+                    // the next set bit's row is read from LDS before the
+                    // current pair's law runs
+                    float4 dn = rowp[mask ? __builtin_ctzll(mask) : 0];
                     while (mask) {
-                        const int bit = __builtin_ctzll(mask);
                         mask &= mask - 1;
-                        const float4 d = rowp[bit];
+                        const float4 d = dn;
+                        dn = rowp[mask ? __builtin_ctzll(mask) : 0];
                         const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         const float inv = __builtin_amdgcn_rcpf(r2);
