@@ -1626,6 +1626,7 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
     const SysDev S = e->sys[0];
     const int n = S.n, n3 = 3 * n;
     const int nck = nsteps / every + 1;
+    const int nadj = (n + ljk::kAdjPerBlock - 1) / ljk::kAdjPerBlock;   // adjoint pair blocks
     const double dt = S.dt, c1 = 1.0 / nsteps;
     const double cT = c1 * 2.0 * density / (3.0 * n);   // d/dv of c1 * rho * T
     const double cTke = 2.0 * density / (3.0 * n);       // rho * T = cTke * KE
@@ -1639,9 +1640,9 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
         (rc = dev_alloc(&seg_vel.p, (size_t)(every + 1) * n3)) || (rc = dev_alloc(&lam_r.p, n3)) ||
         (rc = dev_alloc(&lam_v.p, n3)) || (rc = dev_alloc(&pend.p, n3)) || (rc = dev_alloc(&phi.p, n3)) ||
         (rc = dev_alloc(&jphi.p, n3)) || (rc = dev_alloc(&gradw.p, n3)) || (rc = dev_alloc(&frc.p, n3)) ||
-        (rc = dev_alloc(&part.p, 5 * (size_t)S.nblk)) || (rc = dev_alloc(&acc.p, 2)))
+        (rc = dev_alloc(&part.p, 5 * (size_t)nadj)) || (rc = dev_alloc(&acc.p, 2)))
         return rc;
-    const dim3 gb(S.nblk), tb(kBlock), gv((n3 + kBlock - 1) / kBlock);
+    const dim3 gb(S.nblk), tb(kBlock), gv((n3 + kBlock - 1) / kBlock), ga(nadj);
     auto to_id = [&](double* pos, double* vel, double* disp = nullptr) {
         const ljk::StepArgs p = e->args(e->cur);
         hipLaunchKernelGGL(ljk::k_to_id, gb, tb, 0, e->stream, p, 0, n, pos, vel, disp);
@@ -1668,7 +1669,7 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
         a.gradw = gradw.p;
         a.force = frc.p;
         a.part = part.p;
-        hipLaunchKernelGGL(ljk::k_adjoint_pair, gb, tb, 0, e->stream, a);
+        hipLaunchKernelGGL(ljk::k_adjoint_pair, ga, tb, 0, e->stream, a);
     };
     const bool saved_graphs = e->use_graphs;
     // the displacements and the clock on entry, given back at the end (the
@@ -1693,35 +1694,42 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
     HIP_TRY(hipMemsetAsync(lam_v.p, 0, n3 * sizeof(double), e->stream));
     HIP_TRY(hipMemsetAsync(pend.p, 0, n3 * sizeof(double), e->stream));
     HIP_TRY(hipMemsetAsync(acc.p, 0, 2 * sizeof(double), e->stream));
-    const int nseg = (nsteps + every - 1) / every;
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int t0 = sg * every, t1 = std::min(t0 + every, nsteps);
-        from_id(ck_pos.p + (size_t)sg * n3, ck_vel.p + (size_t)sg * n3);
-        if ((rc = forces_now())) return rc;
+    // one segment of len steps from the checkpoint just loaded: the forces,
+    // the recomputed trajectory, then the reverse sweep over it
+    auto segment = [&](int len) -> int {
+        int r;
+        if ((r = forces_now())) return r;
         to_id(seg_pos.p, seg_vel.p);
-        for (int k = 1; k <= t1 - t0; ++k) {
-            if ((rc = enqueue_step(e))) return rc;
+        for (int k = 1; k <= len; ++k) {
+            if ((r = enqueue_step(e))) return r;
             to_id(seg_pos.p + (size_t)k * n3, seg_vel.p + (size_t)k * n3);
         }
-        if ((rc = enqueue_finalize(e, 0))) return rc;
-        for (int k = t1 - t0; k >= 1; --k) {
+        if ((r = enqueue_finalize(e, 0))) return r;
+        for (int k = len; k >= 1; --k) {
             const double* vt = seg_vel.p + (size_t)k * n3;
             hipLaunchKernelGGL(ljk::k_adj_pre, gv, tb, 0, e->stream, n3, 0.5 * dt, cT, lam_v.p, vt,
                                pend.p, phi.p);
             from_id(seg_pos.p + (size_t)k * n3, vt);
-            if ((rc = enqueue_cells(e, false))) return rc;
+            if ((r = enqueue_cells(e, false))) return r;
             adj_pair(phi.p);
-            hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, S.nblk, c1, cW, eps,
+            hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, nadj, c1, cW, eps,
                                sig, acc.p);
             hipLaunchKernelGGL(ljk::k_adj_post, gv, tb, 0, e->stream, n3, dt, c1, cW, cT, frc.p,
                                gradw.p, jphi.p, vt, lam_r.p, lam_v.p, pend.p);
         }
+        return LJ_OK;
+    };
+    const int nseg = (nsteps + every - 1) / every;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int t0 = sg * every, t1 = std::min(t0 + every, nsteps);
+        from_id(ck_pos.p + (size_t)sg * n3, ck_vel.p + (size_t)sg * n3);
+        if ((rc = segment(t1 - t0))) return rc;
     }
     // ---- t = 0: the first half kick used F(r0)
     from_id(ck_pos.p, ck_vel.p);
     if ((rc = enqueue_cells(e, false))) return rc;
     adj_pair(pend.p);
-    hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, S.nblk, 0.0, 0.0, eps, sig,
+    hipLaunchKernelGGL(ljk::k_adj_acc, dim3(1), tb, 0, e->stream, part.p, nadj, 0.0, 0.0, eps, sig,
                        acc.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));

@@ -2126,21 +2126,46 @@ struct AdjArgs {
     double* part;          // per block: sum df/deps.phi, sum df/dsig.phi, U, W, sum dw/dsig / 2
 };
 
+// One particle per kAdjLanes lanes, lane q walking neighbour cell q of the
+// 27 (reference lattice-image rule, the pair set of for_each_neighbour_fp64),
+// so that a system of a few thousand particles still fills the GPU; the
+// lanes' sums are combined by a fixed butterfly.
+constexpr int kAdjLanes = 32;
+constexpr int kAdjPerBlock = kBlock / kAdjLanes;   // particles per block
+
 __global__ __launch_bounds__(kBlock) void k_adjoint_pair(AdjArgs a)
 {
     const StepArgs& p = a.p;
     const SysDev& S = p.sys[a.sys];
-    const int s = S.atom_off + blockIdx.x * kBlock + threadIdx.x;
-    const bool active = s - S.atom_off < S.n;
-    double acc[5] = {0, 0, 0, 0, 0};
-    if (active) {
-        const int n = a.n;
+    const int q = threadIdx.x & (kAdjLanes - 1);
+    const int g = threadIdx.x / kAdjLanes;
+    const int pi = blockIdx.x * kAdjPerBlock + g;
+    const bool active = pi < S.n;
+    const int s = S.atom_off + (active ? pi : 0);
+    const int n = a.n;
+    double v[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // J phi, grad W, F, 5 sums
+    if (active && q < 27) {
         const int i = p.id[s];
         const double pxi = a.phi[i], pyi = a.phi[n + i], pzi = a.phi[2 * n + i];
-        double jx = 0, jy = 0, jz = 0, wx = 0, wy = 0, wz = 0, fx = 0, fy = 0, fz = 0;
         const double eps = S.eps, sig = sqrt(S.sigma2);
-        for_each_neighbour_fp64(S, p, s, [&](int js, double dx, double dy, double dz, double r2,
-                                             int, int, int) {
+        int cx, cy, cz;
+        cell_xyz_of(p.cell_xyz, p.scell[s], cx, cy, cz);
+        const int ox = q / 9 - 1, oy = (q / 3) % 3 - 1, oz = q % 3 - 1;
+        int nbx, ix, nby, iy, nbz, iz;
+        nb_axis(cx, ox, S.nc[0], nbx, ix);
+        nb_axis(cy, oy, S.nc[1], nby, iy);
+        nb_axis(cz, oz, S.nc[2], nbz, iz);
+        const double shx = ix * S.L[0], shy = iy * S.L[1], shz = iz * S.L[2];
+        const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+        const double xi = p.pos.x[s], yi = p.pos.y[s], zi = p.pos.z[s];
+        const int j1 = p.start[nb + 1];
+        for (int js = p.start[nb]; js < j1; ++js) {
+            if (js == s) continue;
+            const double dx = (xi - p.pos.x[js]) - shx;
+            const double dy = (yi - p.pos.y[js]) - shy;
+            const double dz = (zi - p.pos.z[js]) - shz;
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(r2 < S.rc2)) continue;
             const int j = p.id[js];
             const double inv = 1.0 / r2;
             const double t = S.sigma2 * inv;
@@ -2148,39 +2173,47 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_pair(AdjArgs a)
             const double u = r2 * S.inv_rc2s;
             const double e = eps * (4.0 * s6 * (s6 - 1.0) + S.alpha + S.beta * (u - 1.0));
             const double w = eps * (24.0 * s6 * (2.0 * s6 - 1.0) - 2.0 * S.beta * u);
-            const double g = w * inv;
+            const double gg = w * inv;
             const double wp = eps * (-72.0 * s6 * (4.0 * s6 - 1.0) * inv - 2.0 * S.beta * S.inv_rc2s);
-            const double gp = (wp - g) * inv;
+            const double gp = (wp - gg) * inv;
             const double ddx = pxi - a.phi[j], ddy = pyi - a.phi[n + j], ddz = pzi - a.phi[2 * n + j];
             const double sd = dx * ddx + dy * ddy + dz * ddz;
-            jx += g * ddx + 2.0 * gp * sd * dx;
-            jy += g * ddy + 2.0 * gp * sd * dy;
-            jz += g * ddz + 2.0 * gp * sd * dz;
-            wx += 2.0 * wp * dx; wy += 2.0 * wp * dy; wz += 2.0 * wp * dz;
-            fx += g * dx; fy += g * dy; fz += g * dz;
+            v[0] += gg * ddx + 2.0 * gp * sd * dx;
+            v[1] += gg * ddy + 2.0 * gp * sd * dy;
+            v[2] += gg * ddz + 2.0 * gp * sd * dz;
+            v[3] += 2.0 * wp * dx; v[4] += 2.0 * wp * dy; v[5] += 2.0 * wp * dz;
+            v[6] += gg * dx; v[7] += gg * dy; v[8] += gg * dz;
             const double sphi = dx * pxi + dy * pyi + dz * pzi;
             const double dw_dsig = eps * (144.0 * s6 * (4.0 * s6 - 1.0) + 4.0 * S.beta * u) / sig;
-            acc[0] += (g / eps) * sphi;
-            acc[1] += dw_dsig * inv * sphi;
-            acc[2] += 0.5 * e;
-            acc[3] += 0.5 * w;
-            acc[4] += 0.5 * dw_dsig;
-        });
-        a.jphi[i] = jx; a.jphi[n + i] = jy; a.jphi[2 * n + i] = jz;
-        a.gradw[i] = wx; a.gradw[n + i] = wy; a.gradw[2 * n + i] = wz;
-        a.force[i] = fx; a.force[n + i] = fy; a.force[2 * n + i] = fz;
+            v[9] += (gg / eps) * sphi;
+            v[10] += dw_dsig * inv * sphi;
+            v[11] += 0.5 * e;
+            v[12] += 0.5 * w;
+            v[13] += 0.5 * dw_dsig;
+        }
     }
-    __shared__ double red[5][kBlock / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = 0; k < 5; ++k) {
-        double v = wave_sum(acc[k]);
-        if (lane == 0) red[k][wid] = v;
+    // every lane of the particle ends with the same sums (each butterfly
+    // stage adds the same two values on both sides)
+#pragma unroll
+    for (int k = 0; k < 14; ++k)
+#pragma unroll
+        for (int m = kAdjLanes / 2; m > 0; m >>= 1) v[k] += __shfl_xor(v[k], m, kAdjLanes);
+    if (active && q == 0) {
+        const int i = p.id[s];
+        a.jphi[i] = v[0]; a.jphi[n + i] = v[1]; a.jphi[2 * n + i] = v[2];
+        a.gradw[i] = v[3]; a.gradw[n + i] = v[4]; a.gradw[2 * n + i] = v[5];
+        a.force[i] = v[6]; a.force[n + i] = v[7]; a.force[2 * n + i] = v[8];
+    }
+    __shared__ double red[5][kAdjPerBlock];
+    if (q == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) red[k][g] = v[9 + k];
     }
     __syncthreads();
     if (threadIdx.x < 5) {
-        double v = 0;
-        for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
-        a.part[5 * blockIdx.x + threadIdx.x] = v;
+        double t = 0;
+        for (int w = 0; w < kAdjPerBlock; ++w) t += red[threadIdx.x][w];
+        a.part[5 * blockIdx.x + threadIdx.x] = t;
     }
 }
 
