@@ -120,10 +120,12 @@ struct lj_engine {
     int pair_variant = LJ_PAIR_CELL_TILED;
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
+    int n_lanes = 0;                // ... from the cell-lane kernel
     double split_occupancy = 6.0;   // LJ_PAIR_BY_OCCUPANCY: tiled at >= this many particles per cell
     int* d_cblk_list = nullptr;     // cell-space blocks of the tiled systems (when split)
     int* d_blk_list = nullptr;      // particle-space blocks of the other systems (when split)
-    int n_cblk_list = 0, n_blk_list = 0;
+    int* d_lblk_list = nullptr;     // cell-space blocks of the cell-lane systems (when split)
+    int n_cblk_list = 0, n_blk_list = 0, n_lblk_list = 0;
 
     SysDev* d_sys = nullptr;
     int* d_blk_sys = nullptr;
@@ -218,6 +220,8 @@ struct lj_engine {
         p.skip_tiled = 0;
         p.cblk_list = nullptr;
         p.blk_list = nullptr;
+        p.lblk_list = nullptr;
+        p.cpb = precision == LJ_PRECISION_MIXED ? cells_per_block : ljk::kCellsPerBlockFp64;
         p.pending = d_pending;
         p.cell_xyz = d_cell_xyz;
         p.ncount = d_ncount;
@@ -414,6 +418,20 @@ void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
         launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockFp64), p);
 }
 
+template <int PREC, int LAW, int KICK>
+void launch_pair_lanes(lj_engine* e, const ljk::StepArgs& p)
+{
+    const int grid = (p.lblk_list ? e->n_lblk_list : e->ncblk) * p.cpb;
+    launch(e, ljk::k_pair_cell_lanes<PREC, LAW, KICK>, dim3(grid), dim3(ljk::kLaneBlock), p);
+}
+
+template <int LAW, int KICK>
+void launch_lanes(lj_engine* e, const ljk::StepArgs& p, bool mixed)
+{
+    if (mixed) launch_pair_lanes<ljk::MIXED, LAW, KICK>(e, p);
+    else launch_pair_lanes<ljk::FP64, LAW, KICK>(e, p);
+}
+
 template <int LAW, int KICK>
 void launch_tiled(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 {
@@ -465,20 +483,40 @@ int enqueue_pair(lj_engine* e, int kick, bool measure_only)
             else launch_particle<ljk::LAW_CONSTANT, 0>(e, p, mixed);
         }
     };
-    // with both kernels in use, each launches only over its own systems' blocks
-    const bool split = e->n_tiled > 0 && e->n_tiled < e->nsys;
+    auto lanes = [&] {
+        if (law == ljk::LAW_LJ) {
+            if (kick == 2) launch_lanes<ljk::LAW_LJ, 2>(e, p, mixed);
+            else if (kick) launch_lanes<ljk::LAW_LJ, 1>(e, p, mixed);
+            else launch_lanes<ljk::LAW_LJ, 0>(e, p, mixed);
+        } else {
+            if (kick == 2) launch_lanes<ljk::LAW_CONSTANT, 2>(e, p, mixed);
+            else if (kick) launch_lanes<ljk::LAW_CONSTANT, 1>(e, p, mixed);
+            else launch_lanes<ljk::LAW_CONSTANT, 0>(e, p, mixed);
+        }
+    };
+    // with several kernels in use, each launches only over its own systems'
+    // blocks; the first one launched is timed as "pair"
+    const int n_particle = e->nsys - e->n_tiled - e->n_lanes;
     int rc = LJ_OK;
+    bool first = true;
     if (e->n_tiled > 0) {
-        if (split) p.cblk_list = e->d_cblk_list;
+        if (e->n_tiled < e->nsys) p.cblk_list = e->d_cblk_list;
         rc = timed(e, 5, "pair", tiled);
         p.cblk_list = nullptr;
+        first = false;
     }
-    if (!rc && e->n_tiled < e->nsys) {
-        if (split) {
+    if (!rc && e->n_lanes > 0) {
+        if (e->n_lanes < e->nsys) p.lblk_list = e->d_lblk_list;
+        rc = first ? timed(e, 5, "pair", lanes) : timed(e, 8, "pair_lanes", lanes);
+        p.lblk_list = nullptr;
+        first = false;
+    }
+    if (!rc && n_particle > 0) {
+        if (n_particle < e->nsys) {
             p.skip_tiled = 1;
             p.blk_list = e->d_blk_list;
         }
-        rc = split ? timed(e, 7, "pair_particle", particle) : timed(e, 5, "pair", particle);
+        rc = first ? timed(e, 5, "pair", particle) : timed(e, 7, "pair_particle", particle);
     }
     return rc;
 }
@@ -662,33 +700,39 @@ int assign_pair_kernels(lj_engine* e, bool force_lists = false);
 // block layout changed); the step graphs are rebuilt on their next use.
 int assign_pair_kernels(lj_engine* e, bool force_lists)
 {
-    int nt = 0;
+    int nt = 0, nl = 0;
     bool changed = false;
     for (SysDev& s : e->sys) {
         int t = e->pair_variant == LJ_PAIR_CELL_TILED ? 1 : 0;
+        if (e->pair_variant == LJ_PAIR_CELL_LANES) t = 2;
         if (e->pair_variant == LJ_PAIR_BY_OCCUPANCY)
             t = (s.ncell > 0 && (double)s.n >= e->split_occupancy * (double)s.ncell) ? 1 : 0;
         if (!e->has_cells || s.law == ljk::LAW_NONE) t = 0;
         changed |= t != s.tiled;
         s.tiled = t;
-        nt += t;
+        nt += t == 1;
+        nl += t == 2;
     }
     e->n_tiled = nt;
+    e->n_lanes = nl;
     if (changed || force_lists) {
         drop_graphs(e);
-        std::vector<int> cl, bl;
+        std::vector<int> cl, ll, bl;
         for (const SysDev& s : e->sys) {
             if (s.tiled)
-                for (int b = 0; b < s.ncblk; ++b) cl.push_back(s.cblk_first + b);
+                for (int b = 0; b < s.ncblk; ++b) (s.tiled == 1 ? cl : ll).push_back(s.cblk_first + b);
             else
                 for (int b = 0; b < s.nblk; ++b) bl.push_back(s.blk_first + b);
         }
         e->n_cblk_list = (int)cl.size();
+        e->n_lblk_list = (int)ll.size();
         e->n_blk_list = (int)bl.size();
         HIP_TRY(hipStreamSynchronize(e->stream));
         HIP_TRY(hipMemcpy(e->d_sys, e->sys.data(), e->nsys * sizeof(SysDev), hipMemcpyHostToDevice));
         if (!cl.empty())
             HIP_TRY(hipMemcpy(e->d_cblk_list, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!ll.empty())
+            HIP_TRY(hipMemcpy(e->d_lblk_list, ll.data(), ll.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!bl.empty())
             HIP_TRY(hipMemcpy(e->d_blk_list, bl.data(), bl.size() * sizeof(int), hipMemcpyHostToDevice));
     }
@@ -852,7 +896,10 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         e->sys.push_back(s);
     }
     e->nslot = atom_off;
-    for (const SysDev& s : e->sys) e->n_tiled += s.tiled;
+    for (const SysDev& s : e->sys) {
+        e->n_tiled += s.tiled == 1;
+        e->n_lanes += s.tiled == 2;
+    }
     e->nblk = blk;
     e->cap_cblk = cblk;
     e->cap_sblk = sblk;
@@ -888,6 +935,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_partial, 4 * (size_t)blk)) ||
         (rc = dev_alloc(&e->d_partial_c, 4 * (size_t)std::max(1, cell_off))) ||
         (rc = dev_alloc(&e->d_cblk_list, (size_t)cblk)) || (rc = dev_alloc(&e->d_blk_list, (size_t)blk)) ||
+        (rc = dev_alloc(&e->d_lblk_list, (size_t)cblk)) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
         (rc = dev_alloc(&e->d_pending, nsys)) ||
@@ -972,7 +1020,7 @@ int lj_engine_destroy(lj_engine* e)
     drop_graphs(e);
     void* bufs[] = {e->d_sys, e->d_blk_sys, e->d_cblk_sys, e->d_sblk_sys, e->d_force, e->d_members, e->d_ovf, e->d_novf,
                     e->d_scell, e->d_lpos, e->d_count, e->d_count_init, e->d_cell_xyz, e->d_start, e->d_tile,
-                    e->d_partial, e->d_partial_c, e->d_cblk_list, e->d_blk_list, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
+                    e->d_partial, e->d_partial_c, e->d_cblk_list, e->d_blk_list, e->d_lblk_list, e->d_time, e->d_pe, e->d_vir, e->d_step, e->d_pending, e->d_partial2, e->d_records,
                     e->d_measure, e->d_rdf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -1389,7 +1437,7 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
     int rc = check_engine(e);
     if (rc) return rc;
     if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED &&
-        variant != LJ_PAIR_BY_OCCUPANCY)
+        variant != LJ_PAIR_BY_OCCUPANCY && variant != LJ_PAIR_CELL_LANES)
         return fail(LJ_E_INVALID, "unknown pair kernel variant");
     e->pair_variant = variant;
     return assign_pair_kernels(e);
@@ -1435,7 +1483,8 @@ int lj_engine_pair_kernels(lj_engine* e, int* per_system)
     if (rc) return rc;
     if (!per_system) return fail(LJ_E_INVALID, "null output");
     for (int k = 0; k < e->nsys; ++k)
-        per_system[k] = e->sys[k].tiled ? LJ_PAIR_CELL_TILED : LJ_PAIR_PER_PARTICLE;
+        per_system[k] = e->sys[k].tiled == 1 ? LJ_PAIR_CELL_TILED
+                        : e->sys[k].tiled == 2 ? LJ_PAIR_CELL_LANES : LJ_PAIR_PER_PARTICLE;
     return LJ_OK;
 }
 
@@ -1511,13 +1560,14 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         return code;
     };
     if (save() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: save failed"), true);
-    // candidates: all tiled, all per-particle, and splits by cell occupancy
+    // candidates: all tiled, all per-particle, all cell-lane, and splits by cell occupancy
     // whose tiled sets differ from those and from each other
     struct Candidate {
         int variant;
         double occupancy;
     };
-    std::vector<Candidate> cands = {{LJ_PAIR_CELL_TILED, 0.0}, {LJ_PAIR_PER_PARTICLE, 0.0}};
+    std::vector<Candidate> cands = {{LJ_PAIR_CELL_TILED, 0.0}, {LJ_PAIR_PER_PARTICLE, 0.0},
+                                    {LJ_PAIR_CELL_LANES, 0.0}};
     {
         std::vector<int> seen = {0, e->nsys};
         for (double thr : {1.5, 3.0, 6.0, 12.0}) {

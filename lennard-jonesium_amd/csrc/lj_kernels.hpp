@@ -60,7 +60,7 @@ struct SysDev {
     int cblk_first;    // first cell-space block (the tiled kernel's cells per block each)
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
-    int tiled;         // pair forces from the tiled kernel (1) or the per-particle kernel (0)
+    int tiled;         // pair forces from the tiled (1), cell-lane (2) or per-particle (0) kernel
     int sort_k;        // cells per wave in the sort (about 44 particles per wave)
     int sblk_first;    // first sort block (kBlock / 64 waves of sort_k cells each)
     double L[3];       // box
@@ -226,6 +226,8 @@ struct StepArgs {
     int skip_tiled;                     // per-particle pair kernel: leave tiled systems alone
     const int* cblk_list;               // tiled pair kernels: cell-space blocks to run (null: all)
     const int* blk_list;                // per-particle pair kernel: blocks to run (null: all)
+    const int* lblk_list;               // cell-lane pair kernel: cell-space blocks to run (null: all)
+    int cpb;                            // cells per cell-space block
     int* pending;                       // per system: a step's block sums await the finalize
     const int* cell_xyz;                // per cell: x | y << 10 | z << 20 (cell coordinates)
     int* ncount;                        // optional: in-cutoff neighbours per slot
@@ -1085,6 +1087,204 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 }
 
 
+// ---------------------------------------------------------------------------
+// 6a. cell-lane pair kernel, for small systems: one workgroup per home cell,
+//     kLaneGroup lanes per home particle, lane q walking neighbour cell q of
+//     the 27 (the reference lattice-image rule; the pair set, test and law of
+//     the per-particle kernel, in either precision).  A system of a few
+//     thousand particles has too few cells to keep a wave per cell busy and
+//     too few particles to hide a thread's walk over 27 cells; here a lane
+//     walks one cell.  The lanes of a particle combine their sums by a fixed
+//     butterfly, and the cell's record (sum |v|^2, PE, W, sum |d|^2) adds its
+//     particles' in slot order, one record per cell as the tiled kernels
+//     write.  The grid runs cpb workgroups per cell-space block.
+// ---------------------------------------------------------------------------
+constexpr int kLaneGroup = 32;                            // 27 lanes walk a cell
+constexpr int kLaneBlock = 1024;
+constexpr int kLaneParticles = kLaneBlock / kLaneGroup;   // home particles per pass
+
+template <int PREC, int LAW, int KICK>
+__global__ __launch_bounds__(kLaneBlock) void k_pair_cell_lanes(StepArgs p)
+{
+    const int cb = (int)blockIdx.x / p.cpb;
+    const int b = p.lblk_list ? p.lblk_list[cb] : cb;
+    const int k = p.cblk_sys[b];
+    const SysDev& S = p.sys[k];
+    if (S.tiled != 2) return;   // this system's forces come from another kernel
+    const int lc = (b - S.cblk_first) * p.cpb + (int)blockIdx.x % p.cpb;
+    if (lc >= S.ncell) return;   // past the system's last cell (no barrier yet)
+    const int c = S.cell_off + lc;
+    const int q = threadIdx.x & (kLaneGroup - 1);
+    const int gi = threadIdx.x / kLaneGroup;
+    __shared__ double red[kLaneParticles][4];
+    double cacc[4] = {0, 0, 0, 0};   // the cell's sums (thread 0)
+    if (!KICK || S.active) {
+        const int h0 = p.start[c], h1 = p.start[c + 1];
+        int cx, cy, cz;
+        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+        // this lane's neighbour cell: its slot range and image shift
+        int j0 = 0, j1 = 0;
+        double shx = 0., shy = 0., shz = 0.;
+        float fsx = 0.f, fsy = 0.f, fsz = 0.f;
+        if (q < 27) {
+            const int ox = q / 9 - 1, oy = (q / 3) % 3 - 1, oz = q % 3 - 1;
+            int nbx, ix, nby, iy, nbz, iz;
+            nb_axis(cx, ox, S.nc[0], nbx, ix);
+            nb_axis(cy, oy, S.nc[1], nby, iy);
+            nb_axis(cz, oz, S.nc[2], nbz, iz);
+            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
+            j0 = p.start[nb];
+            j1 = p.start[nb + 1];
+            shx = ix * S.L[0]; shy = iy * S.L[1]; shz = iz * S.L[2];
+            fsx = S.shf[0][3 * (ox + 1) + ix + 1];
+            fsy = S.shf[1][3 * (oy + 1) + iy + 1];
+            fsz = S.shf[2][3 * (oz + 1) + iz + 1];
+        }
+        const int last = p.nslot - 1;
+        for (int i0 = h0; i0 < h1; i0 += kLaneParticles) {   // uniform over the workgroup
+            const int s = i0 + gi;
+            const bool act = s < h1;
+            const int sc = act ? s : h0;
+            double fx = 0., fy = 0., fz = 0., pe = 0., w = 0.;
+            int nct = 0;
+            if (PREC == FP64) {
+                const double xi = p.pos.x[sc], yi = p.pos.y[sc], zi = p.pos.z[sc];
+                const double rc2 = S.rc2;
+                for (int t0 = j0; t0 < j1; t0 += kPairBatch64) {
+                    double xb[kPairBatch64], yb[kPairBatch64], zb[kPairBatch64];
+#pragma unroll
+                    for (int u = 0; u < kPairBatch64; ++u) {
+                        const int jc = min(t0 + u, last);
+                        xb[u] = p.pos.x[jc]; yb[u] = p.pos.y[jc]; zb[u] = p.pos.z[jc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPairBatch64; ++u) {
+                        const int j = t0 + u;
+                        if (j < j1 && j != sc) {
+                            const double dx = (xi - xb[u]) - shx;
+                            const double dy = (yi - yb[u]) - shy;
+                            const double dz = (zi - zb[u]) - shz;
+                            const double r2 = (dx * dx + dy * dy) + dz * dz;
+                            if (r2 < rc2) {
+                                pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
+                                ++nct;
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = kLaneGroup / 2; m > 0; m >>= 1) {
+                    fx += __shfl_xor(fx, m, kLaneGroup);
+                    fy += __shfl_xor(fy, m, kLaneGroup);
+                    fz += __shfl_xor(fz, m, kLaneGroup);
+                    pe += __shfl_xor(pe, m, kLaneGroup);
+                    w += __shfl_xor(w, m, kLaneGroup);
+                    nct += __shfl_xor(nct, m, kLaneGroup);
+                }
+            } else {
+                // the law of the per-particle mixed kernel: F / 24 from
+                // g / 24 = s^6 (2 s^6 - 1), float sums of g, s^6 and r^2
+                const float4 li = p.lpos[sc];
+                const float rc2f = S.rc2f, inv_s2 = S.inv_s2f, cw24 = S.cw24f;
+                float ax = 0.f, ay = 0.f, az = 0.f, gs = 0.f, s6s = 0.f, r2s = 0.f;
+                float spe = 0.f, sww = 0.f;
+                for (int t0 = j0; t0 < j1; t0 += kPairBatch) {
+                    float4 lb[kPairBatch];
+#pragma unroll
+                    for (int u = 0; u < kPairBatch; ++u) lb[u] = p.lpos[min(t0 + u, last)];
+#pragma unroll
+                    for (int u = 0; u < kPairBatch; ++u) {
+                        const int j = t0 + u;
+                        if (j < j1 && j != sc) {
+                            const float dx = (li.x - lb[u].x) + fsx;
+                            const float dy = (li.y - lb[u].y) + fsy;
+                            const float dz = (li.z - lb[u].z) + fsz;
+                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            if (r2 < rc2f) {
+                                ++nct;
+                                if (LAW == LAW_LJ) {
+                                    const float inv = __builtin_amdgcn_rcpf(r2);
+                                    const float t = inv * inv_s2;
+                                    const float s6 = t * t * t;
+                                    const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
+                                    gs += g;
+                                    s6s += s6;
+                                    r2s += r2;
+                                    const float fm = __builtin_fmaf(g, inv, cw24);
+                                    ax = __builtin_fmaf(fm, dx, ax);
+                                    ay = __builtin_fmaf(fm, dy, ay);
+                                    az = __builtin_fmaf(fm, dz, az);
+                                } else {
+                                    const float nrm = sqrtf(r2);
+                                    const float vv = (float)S.strength * ((float)S.rc - nrm);
+                                    const float ww = (float)S.strength * nrm;
+                                    const float fm = ww / r2;
+                                    ax += fm * dx; ay += fm * dy; az += fm * dz;
+                                    spe += vv;
+                                    sww += ww;
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = kLaneGroup / 2; m > 0; m >>= 1) {
+                    ax += __shfl_xor(ax, m, kLaneGroup);
+                    ay += __shfl_xor(ay, m, kLaneGroup);
+                    az += __shfl_xor(az, m, kLaneGroup);
+                    gs += __shfl_xor(gs, m, kLaneGroup);
+                    s6s += __shfl_xor(s6s, m, kLaneGroup);
+                    r2s += __shfl_xor(r2s, m, kLaneGroup);
+                    spe += __shfl_xor(spe, m, kLaneGroup);
+                    sww += __shfl_xor(sww, m, kLaneGroup);
+                    nct += __shfl_xor(nct, m, kLaneGroup);
+                }
+                if (LAW == LAW_LJ) {
+                    const double u_sum = (double)r2s * S.inv_rc2s;
+                    pe = S.eps * (2.0 * (double)gs - 2.0 * (double)s6s +
+                                  (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
+                    w = S.eps * (24.0 * (double)gs - 2.0 * S.beta * u_sum);
+                    const double fs = 24.0 * S.eps;
+                    fx = fs * (double)ax; fy = fs * (double)ay; fz = fs * (double)az;
+                } else {
+                    pe = spe;
+                    w = sww;
+                    fx = ax; fy = ay; fz = az;
+                }
+            }
+            double acc[4] = {0, 0, 0, 0};
+            if (act && q == 0) {
+                if (p.ncount) p.ncount[s] = nct;
+                if (KICK != 2) { p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz; }
+                double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+                if (KICK) kick_store(p, S, s, fx, fy, fz, vx, vy, vz, KICK == 2);
+                const double ddx = p.disp.x[s], ddy = p.disp.y[s], ddz = p.disp.z[s];
+                acc[0] = (vx * vx + vy * vy) + vz * vz;
+                acc[1] = (1. / 2.) * pe;
+                acc[2] = (1. / 2.) * w;
+                acc[3] = (ddx * ddx + ddy * ddy) + ddz * ddz;
+            }
+            if (q == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[gi][r] = acc[r];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int np = min(kLaneParticles, h1 - i0);
+                for (int g = 0; g < np; ++g)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) cacc[r] += red[g][r];
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) p.partial_c[4 * (size_t)c + r] = cacc[r];
+        if (KICK && lc == 0) p.pending[k] = 1;   // the step awaits its finalize
+    }
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1206,7 +1406,7 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
     const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
-    if (!S.tiled) return;   // this system's forces come from the per-particle kernel
+    if (S.tiled != 1) return;   // this system's forces come from another kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveLds& W = wl[wid];
@@ -1546,7 +1746,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
     const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
-    if (!S.tiled) return;   // this system's forces come from the per-particle kernel
+    if (S.tiled != 1) return;   // this system's forces come from another kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lc = (b - S.cblk_first) * kCellsPerBlockFp64 + wid;
