@@ -232,23 +232,23 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
  * per system: tiled where the mean cell occupancy N / cells is at least the
  * threshold of lj_engine_set_pair_split (default 6), per-particle elsewhere
  * (dilute replicas of a sweep), both kernels running in the same step;
- * LJ_PAIR_CELL_LANES runs one workgroup per home cell with 32 lanes per home
- * particle, one per neighbour cell (small systems, where a wave per cell or a
- * thread per particle leaves the GPU idle).  All give the same pair set and,
- * in FP64, the same per-pair arithmetic.  No
+ * LJ_PAIR_CELL_MULTIWAVE runs the tiled kernel with four waves per home cell,
+ * each over a quarter of the neighbour cells (small systems, where a wave per
+ * cell leaves most of the GPU idle).  All give the same pair set and, in
+ * FP64, the same cutoff decisions.  No
  * counterpart in the reference (its single pair filter,
  * engine/particle_pair_filter.cpp:131-192, has no variants). */
 #define LJ_PAIR_PER_PARTICLE 0
 #define LJ_PAIR_CELL_TILED 1
 #define LJ_PAIR_BY_OCCUPANCY 2
-#define LJ_PAIR_CELL_LANES 3
+#define LJ_PAIR_CELL_MULTIWAVE 3
 int lj_engine_set_pair_kernel(lj_engine* e, int variant);
 
 /* Selects LJ_PAIR_BY_OCCUPANCY with the given threshold (particles per cell). */
 int lj_engine_set_pair_split(lj_engine* e, double min_occupancy);
 
 /* The kernel each system uses now: per_system[k] = LJ_PAIR_CELL_TILED,
- * LJ_PAIR_CELL_LANES or LJ_PAIR_PER_PARTICLE. */
+ * LJ_PAIR_CELL_MULTIWAVE or LJ_PAIR_PER_PARTICLE. */
 int lj_engine_pair_kernels(lj_engine* e, int* per_system);
 
 /* Launch shape (no reference counterpart; neither knob changes any result):

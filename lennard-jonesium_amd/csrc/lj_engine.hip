@@ -120,11 +120,11 @@ struct lj_engine {
     int pair_variant = LJ_PAIR_CELL_TILED;
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
-    int n_lanes = 0;                // ... from the cell-lane kernel
+    int n_multi = 0;                // ... from the tiled kernel with several waves per cell
     double split_occupancy = 6.0;   // LJ_PAIR_BY_OCCUPANCY: tiled at >= this many particles per cell
     int* d_cblk_list = nullptr;     // cell-space blocks of the tiled systems (when split)
     int* d_blk_list = nullptr;      // particle-space blocks of the other systems (when split)
-    int* d_lblk_list = nullptr;     // cell-space blocks of the cell-lane systems (when split)
+    int* d_lblk_list = nullptr;     // cell-space blocks of the several-wave systems (when split)
     int n_cblk_list = 0, n_blk_list = 0, n_lblk_list = 0;
 
     SysDev* d_sys = nullptr;
@@ -418,18 +418,25 @@ void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
         launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockFp64), p);
 }
 
+// Systems flagged 2 (small systems): the tiled kernel of their precision with
+// several waves per home cell, each over a share of the neighbour cells.
 template <int PREC, int LAW, int KICK>
-void launch_pair_lanes(lj_engine* e, const ljk::StepArgs& p)
+void launch_pair_multiwave(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = (p.lblk_list ? e->n_lblk_list : e->ncblk) * p.cpb;
-    launch(e, ljk::k_pair_cell_lanes<PREC, LAW, KICK>, dim3(grid), dim3(ljk::kLaneBlock), p);
+    if constexpr (PREC == ljk::MIXED)
+        launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 1, ljk::kWavesPerCellSmall>, dim3(grid),
+               dim3(64 * ljk::kWavesPerCellSmall), p);
+    else
+        launch(e, ljk::k_pair_cells_fp64<LAW, KICK, ljk::kWavesPerCellSmall>, dim3(grid),
+               dim3(64 * ljk::kWavesPerCellSmall), p);
 }
 
 template <int LAW, int KICK>
-void launch_lanes(lj_engine* e, const ljk::StepArgs& p, bool mixed)
+void launch_multiwave(lj_engine* e, const ljk::StepArgs& p, bool mixed)
 {
-    if (mixed) launch_pair_lanes<ljk::MIXED, LAW, KICK>(e, p);
-    else launch_pair_lanes<ljk::FP64, LAW, KICK>(e, p);
+    if (mixed) launch_pair_multiwave<ljk::MIXED, LAW, KICK>(e, p);
+    else launch_pair_multiwave<ljk::FP64, LAW, KICK>(e, p);
 }
 
 template <int LAW, int KICK>
@@ -483,20 +490,20 @@ int enqueue_pair(lj_engine* e, int kick, bool measure_only)
             else launch_particle<ljk::LAW_CONSTANT, 0>(e, p, mixed);
         }
     };
-    auto lanes = [&] {
+    auto multiwave = [&] {
         if (law == ljk::LAW_LJ) {
-            if (kick == 2) launch_lanes<ljk::LAW_LJ, 2>(e, p, mixed);
-            else if (kick) launch_lanes<ljk::LAW_LJ, 1>(e, p, mixed);
-            else launch_lanes<ljk::LAW_LJ, 0>(e, p, mixed);
+            if (kick == 2) launch_multiwave<ljk::LAW_LJ, 2>(e, p, mixed);
+            else if (kick) launch_multiwave<ljk::LAW_LJ, 1>(e, p, mixed);
+            else launch_multiwave<ljk::LAW_LJ, 0>(e, p, mixed);
         } else {
-            if (kick == 2) launch_lanes<ljk::LAW_CONSTANT, 2>(e, p, mixed);
-            else if (kick) launch_lanes<ljk::LAW_CONSTANT, 1>(e, p, mixed);
-            else launch_lanes<ljk::LAW_CONSTANT, 0>(e, p, mixed);
+            if (kick == 2) launch_multiwave<ljk::LAW_CONSTANT, 2>(e, p, mixed);
+            else if (kick) launch_multiwave<ljk::LAW_CONSTANT, 1>(e, p, mixed);
+            else launch_multiwave<ljk::LAW_CONSTANT, 0>(e, p, mixed);
         }
     };
     // with several kernels in use, each launches only over its own systems'
     // blocks; the first one launched is timed as "pair"
-    const int n_particle = e->nsys - e->n_tiled - e->n_lanes;
+    const int n_particle = e->nsys - e->n_tiled - e->n_multi;
     int rc = LJ_OK;
     bool first = true;
     if (e->n_tiled > 0) {
@@ -505,9 +512,9 @@ int enqueue_pair(lj_engine* e, int kick, bool measure_only)
         p.cblk_list = nullptr;
         first = false;
     }
-    if (!rc && e->n_lanes > 0) {
-        if (e->n_lanes < e->nsys) p.lblk_list = e->d_lblk_list;
-        rc = first ? timed(e, 5, "pair", lanes) : timed(e, 8, "pair_lanes", lanes);
+    if (!rc && e->n_multi > 0) {
+        if (e->n_multi < e->nsys) p.lblk_list = e->d_lblk_list;
+        rc = first ? timed(e, 5, "pair", multiwave) : timed(e, 8, "pair_multiwave", multiwave);
         p.lblk_list = nullptr;
         first = false;
     }
@@ -704,7 +711,7 @@ int assign_pair_kernels(lj_engine* e, bool force_lists)
     bool changed = false;
     for (SysDev& s : e->sys) {
         int t = e->pair_variant == LJ_PAIR_CELL_TILED ? 1 : 0;
-        if (e->pair_variant == LJ_PAIR_CELL_LANES) t = 2;
+        if (e->pair_variant == LJ_PAIR_CELL_MULTIWAVE) t = 2;
         if (e->pair_variant == LJ_PAIR_BY_OCCUPANCY)
             t = (s.ncell > 0 && (double)s.n >= e->split_occupancy * (double)s.ncell) ? 1 : 0;
         if (!e->has_cells || s.law == ljk::LAW_NONE) t = 0;
@@ -714,7 +721,7 @@ int assign_pair_kernels(lj_engine* e, bool force_lists)
         nl += t == 2;
     }
     e->n_tiled = nt;
-    e->n_lanes = nl;
+    e->n_multi = nl;
     if (changed || force_lists) {
         drop_graphs(e);
         std::vector<int> cl, ll, bl;
@@ -898,7 +905,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
     e->nslot = atom_off;
     for (const SysDev& s : e->sys) {
         e->n_tiled += s.tiled == 1;
-        e->n_lanes += s.tiled == 2;
+        e->n_multi += s.tiled == 2;
     }
     e->nblk = blk;
     e->cap_cblk = cblk;
@@ -1437,7 +1444,7 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
     int rc = check_engine(e);
     if (rc) return rc;
     if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED &&
-        variant != LJ_PAIR_BY_OCCUPANCY && variant != LJ_PAIR_CELL_LANES)
+        variant != LJ_PAIR_BY_OCCUPANCY && variant != LJ_PAIR_CELL_MULTIWAVE)
         return fail(LJ_E_INVALID, "unknown pair kernel variant");
     e->pair_variant = variant;
     return assign_pair_kernels(e);
@@ -1484,7 +1491,7 @@ int lj_engine_pair_kernels(lj_engine* e, int* per_system)
     if (!per_system) return fail(LJ_E_INVALID, "null output");
     for (int k = 0; k < e->nsys; ++k)
         per_system[k] = e->sys[k].tiled == 1 ? LJ_PAIR_CELL_TILED
-                        : e->sys[k].tiled == 2 ? LJ_PAIR_CELL_LANES : LJ_PAIR_PER_PARTICLE;
+                        : e->sys[k].tiled == 2 ? LJ_PAIR_CELL_MULTIWAVE : LJ_PAIR_PER_PARTICLE;
     return LJ_OK;
 }
 
@@ -1560,14 +1567,15 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         return code;
     };
     if (save() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: save failed"), true);
-    // candidates: all tiled, all per-particle, all cell-lane, and splits by cell occupancy
+    // candidates: all tiled, all per-particle, all several-wave tiled, and
+    // splits by cell occupancy
     // whose tiled sets differ from those and from each other
     struct Candidate {
         int variant;
         double occupancy;
     };
     std::vector<Candidate> cands = {{LJ_PAIR_CELL_TILED, 0.0}, {LJ_PAIR_PER_PARTICLE, 0.0},
-                                    {LJ_PAIR_CELL_LANES, 0.0}};
+                                    {LJ_PAIR_CELL_MULTIWAVE, 0.0}};
     {
         std::vector<int> seen = {0, e->nsys};
         for (double thr : {1.5, 3.0, 6.0, 12.0}) {

@@ -39,6 +39,7 @@ constexpr int kBlock = 256;        // slots per block in particle-space kernels
 constexpr int kScanTile = 1024;    // cells per block in the scan kernels
 constexpr int kCellsPerBlockMixed = 4;  // default home cells (one wave64 each) per block, mixed tiled kernel
 constexpr int kCellsPerBlockFp64 = 1;   // FP64 tiled kernel: single-wave blocks retire independently
+constexpr int kWavesPerCellSmall = 4;   // tiled kernels for small systems: waves per home cell
 constexpr int kHistBins = 1024;    // LDS histogram window of the cell count (cells)
 constexpr int kMaxParts = 64;      // first-level parts per system of the record sums
 constexpr int kPartSums = 1024;    // partial sums per first-level part (below kMaxParts parts)
@@ -60,7 +61,8 @@ struct SysDev {
     int cblk_first;    // first cell-space block (the tiled kernel's cells per block each)
     int ncblk;         // cell-space blocks of this system
     int active;        // 0: frozen (no integration, forces or records; cells still rebuilt)
-    int tiled;         // pair forces from the tiled (1), cell-lane (2) or per-particle (0) kernel
+    int tiled;         // pair forces from the tiled kernel with one wave (1) or several waves (2) per
+                       // home cell, or from the per-particle kernel (0)
     int sort_k;        // cells per wave in the sort (about 44 particles per wave)
     int sblk_first;    // first sort block (kBlock / 64 waves of sort_k cells each)
     double L[3];       // box
@@ -226,7 +228,7 @@ struct StepArgs {
     int skip_tiled;                     // per-particle pair kernel: leave tiled systems alone
     const int* cblk_list;               // tiled pair kernels: cell-space blocks to run (null: all)
     const int* blk_list;                // per-particle pair kernel: blocks to run (null: all)
-    const int* lblk_list;               // cell-lane pair kernel: cell-space blocks to run (null: all)
+    const int* lblk_list;               // several waves per cell (small systems): blocks to run (null: all)
     int cpb;                            // cells per cell-space block
     int* pending;                       // per system: a step's block sums await the finalize
     const int* cell_xyz;                // per cell: x | y << 10 | z << 20 (cell coordinates)
@@ -1087,204 +1089,6 @@ __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
 }
 
 
-// ---------------------------------------------------------------------------
-// 6a. cell-lane pair kernel, for small systems: one workgroup per home cell,
-//     kLaneGroup lanes per home particle, lane q walking neighbour cell q of
-//     the 27 (the reference lattice-image rule; the pair set, test and law of
-//     the per-particle kernel, in either precision).  A system of a few
-//     thousand particles has too few cells to keep a wave per cell busy and
-//     too few particles to hide a thread's walk over 27 cells; here a lane
-//     walks one cell.  The lanes of a particle combine their sums by a fixed
-//     butterfly, and the cell's record (sum |v|^2, PE, W, sum |d|^2) adds its
-//     particles' in slot order, one record per cell as the tiled kernels
-//     write.  The grid runs cpb workgroups per cell-space block.
-// ---------------------------------------------------------------------------
-constexpr int kLaneGroup = 32;                            // 27 lanes walk a cell
-constexpr int kLaneBlock = 1024;
-constexpr int kLaneParticles = kLaneBlock / kLaneGroup;   // home particles per pass
-
-template <int PREC, int LAW, int KICK>
-__global__ __launch_bounds__(kLaneBlock) void k_pair_cell_lanes(StepArgs p)
-{
-    const int cb = (int)blockIdx.x / p.cpb;
-    const int b = p.lblk_list ? p.lblk_list[cb] : cb;
-    const int k = p.cblk_sys[b];
-    const SysDev& S = p.sys[k];
-    if (S.tiled != 2) return;   // this system's forces come from another kernel
-    const int lc = (b - S.cblk_first) * p.cpb + (int)blockIdx.x % p.cpb;
-    if (lc >= S.ncell) return;   // past the system's last cell (no barrier yet)
-    const int c = S.cell_off + lc;
-    const int q = threadIdx.x & (kLaneGroup - 1);
-    const int gi = threadIdx.x / kLaneGroup;
-    __shared__ double red[kLaneParticles][4];
-    double cacc[4] = {0, 0, 0, 0};   // the cell's sums (thread 0)
-    if (!KICK || S.active) {
-        const int h0 = p.start[c], h1 = p.start[c + 1];
-        int cx, cy, cz;
-        cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
-        // this lane's neighbour cell: its slot range and image shift
-        int j0 = 0, j1 = 0;
-        double shx = 0., shy = 0., shz = 0.;
-        float fsx = 0.f, fsy = 0.f, fsz = 0.f;
-        if (q < 27) {
-            const int ox = q / 9 - 1, oy = (q / 3) % 3 - 1, oz = q % 3 - 1;
-            int nbx, ix, nby, iy, nbz, iz;
-            nb_axis(cx, ox, S.nc[0], nbx, ix);
-            nb_axis(cy, oy, S.nc[1], nby, iy);
-            nb_axis(cz, oz, S.nc[2], nbz, iz);
-            const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
-            j0 = p.start[nb];
-            j1 = p.start[nb + 1];
-            shx = ix * S.L[0]; shy = iy * S.L[1]; shz = iz * S.L[2];
-            fsx = S.shf[0][3 * (ox + 1) + ix + 1];
-            fsy = S.shf[1][3 * (oy + 1) + iy + 1];
-            fsz = S.shf[2][3 * (oz + 1) + iz + 1];
-        }
-        const int last = p.nslot - 1;
-        for (int i0 = h0; i0 < h1; i0 += kLaneParticles) {   // uniform over the workgroup
-            const int s = i0 + gi;
-            const bool act = s < h1;
-            const int sc = act ? s : h0;
-            double fx = 0., fy = 0., fz = 0., pe = 0., w = 0.;
-            int nct = 0;
-            if (PREC == FP64) {
-                const double xi = p.pos.x[sc], yi = p.pos.y[sc], zi = p.pos.z[sc];
-                const double rc2 = S.rc2;
-                for (int t0 = j0; t0 < j1; t0 += kPairBatch64) {
-                    double xb[kPairBatch64], yb[kPairBatch64], zb[kPairBatch64];
-#pragma unroll
-                    for (int u = 0; u < kPairBatch64; ++u) {
-                        const int jc = min(t0 + u, last);
-                        xb[u] = p.pos.x[jc]; yb[u] = p.pos.y[jc]; zb[u] = p.pos.z[jc];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kPairBatch64; ++u) {
-                        const int j = t0 + u;
-                        if (j < j1 && j != sc) {
-                            const double dx = (xi - xb[u]) - shx;
-                            const double dy = (yi - yb[u]) - shy;
-                            const double dz = (zi - zb[u]) - shz;
-                            const double r2 = (dx * dx + dy * dy) + dz * dz;
-                            if (r2 < rc2) {
-                                pair_fp64<LAW>(S, dx, dy, dz, r2, fx, fy, fz, pe, w);
-                                ++nct;
-                            }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int m = kLaneGroup / 2; m > 0; m >>= 1) {
-                    fx += __shfl_xor(fx, m, kLaneGroup);
-                    fy += __shfl_xor(fy, m, kLaneGroup);
-                    fz += __shfl_xor(fz, m, kLaneGroup);
-                    pe += __shfl_xor(pe, m, kLaneGroup);
-                    w += __shfl_xor(w, m, kLaneGroup);
-                    nct += __shfl_xor(nct, m, kLaneGroup);
-                }
-            } else {
-                // the law of the per-particle mixed kernel: F / 24 from
-                // g / 24 = s^6 (2 s^6 - 1), float sums of g, s^6 and r^2
-                const float4 li = p.lpos[sc];
-                const float rc2f = S.rc2f, inv_s2 = S.inv_s2f, cw24 = S.cw24f;
-                float ax = 0.f, ay = 0.f, az = 0.f, gs = 0.f, s6s = 0.f, r2s = 0.f;
-                float spe = 0.f, sww = 0.f;
-                for (int t0 = j0; t0 < j1; t0 += kPairBatch) {
-                    float4 lb[kPairBatch];
-#pragma unroll
-                    for (int u = 0; u < kPairBatch; ++u) lb[u] = p.lpos[min(t0 + u, last)];
-#pragma unroll
-                    for (int u = 0; u < kPairBatch; ++u) {
-                        const int j = t0 + u;
-                        if (j < j1 && j != sc) {
-                            const float dx = (li.x - lb[u].x) + fsx;
-                            const float dy = (li.y - lb[u].y) + fsy;
-                            const float dz = (li.z - lb[u].z) + fsz;
-                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                            if (r2 < rc2f) {
-                                ++nct;
-                                if (LAW == LAW_LJ) {
-                                    const float inv = __builtin_amdgcn_rcpf(r2);
-                                    const float t = inv * inv_s2;
-                                    const float s6 = t * t * t;
-                                    const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
-                                    gs += g;
-                                    s6s += s6;
-                                    r2s += r2;
-                                    const float fm = __builtin_fmaf(g, inv, cw24);
-                                    ax = __builtin_fmaf(fm, dx, ax);
-                                    ay = __builtin_fmaf(fm, dy, ay);
-                                    az = __builtin_fmaf(fm, dz, az);
-                                } else {
-                                    const float nrm = sqrtf(r2);
-                                    const float vv = (float)S.strength * ((float)S.rc - nrm);
-                                    const float ww = (float)S.strength * nrm;
-                                    const float fm = ww / r2;
-                                    ax += fm * dx; ay += fm * dy; az += fm * dz;
-                                    spe += vv;
-                                    sww += ww;
-                                }
-                            }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int m = kLaneGroup / 2; m > 0; m >>= 1) {
-                    ax += __shfl_xor(ax, m, kLaneGroup);
-                    ay += __shfl_xor(ay, m, kLaneGroup);
-                    az += __shfl_xor(az, m, kLaneGroup);
-                    gs += __shfl_xor(gs, m, kLaneGroup);
-                    s6s += __shfl_xor(s6s, m, kLaneGroup);
-                    r2s += __shfl_xor(r2s, m, kLaneGroup);
-                    spe += __shfl_xor(spe, m, kLaneGroup);
-                    sww += __shfl_xor(sww, m, kLaneGroup);
-                    nct += __shfl_xor(nct, m, kLaneGroup);
-                }
-                if (LAW == LAW_LJ) {
-                    const double u_sum = (double)r2s * S.inv_rc2s;
-                    pe = S.eps * (2.0 * (double)gs - 2.0 * (double)s6s +
-                                  (S.alpha - S.beta) * (double)nct + S.beta * u_sum);
-                    w = S.eps * (24.0 * (double)gs - 2.0 * S.beta * u_sum);
-                    const double fs = 24.0 * S.eps;
-                    fx = fs * (double)ax; fy = fs * (double)ay; fz = fs * (double)az;
-                } else {
-                    pe = spe;
-                    w = sww;
-                    fx = ax; fy = ay; fz = az;
-                }
-            }
-            double acc[4] = {0, 0, 0, 0};
-            if (act && q == 0) {
-                if (p.ncount) p.ncount[s] = nct;
-                if (KICK != 2) { p.force.x[s] = fx; p.force.y[s] = fy; p.force.z[s] = fz; }
-                double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
-                if (KICK) kick_store(p, S, s, fx, fy, fz, vx, vy, vz, KICK == 2);
-                const double ddx = p.disp.x[s], ddy = p.disp.y[s], ddz = p.disp.z[s];
-                acc[0] = (vx * vx + vy * vy) + vz * vz;
-                acc[1] = (1. / 2.) * pe;
-                acc[2] = (1. / 2.) * w;
-                acc[3] = (ddx * ddx + ddy * ddy) + ddz * ddz;
-            }
-            if (q == 0) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) red[gi][r] = acc[r];
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const int np = min(kLaneParticles, h1 - i0);
-                for (int g = 0; g < np; ++g)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) cacc[r] += red[g][r];
-            }
-            __syncthreads();
-        }
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) p.partial_c[4 * (size_t)c + r] = cacc[r];
-        if (KICK && lc == 0) p.pending[k] = 1;   // the step awaits its finalize
-    }
-}
-
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1388,9 +1192,14 @@ struct SliceGeomTable {
 __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 
 
-template <int LAW, int KICK, int NW>
-__global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
+// NWC > 1 (small systems, flag tiled 2): one home cell per workgroup of NWC
+// waves, wave sub taking the neighbour cells q with q % NWC == sub; their
+// partial forces are added in wave order before the kick.  The grid then runs
+// cpb workgroups per cell-space block.
+template <int LAW, int KICK, int NW, int NWC = 1>
+__global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs p)
 {
+    static_assert(NW == 1 || NWC == 1, "several waves per cell: one cell per block");
     // One record per wave, so that every per-lane LDS address is one base
     // register plus an immediate offset.
     struct alignas(16) WaveLds {
@@ -1401,17 +1210,27 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
         int tb_off[32];      // entries 27..31 = INT_MAX (search padding)
         int selfe[64];       // staged entry of each chunk particle (-1: none)
     };
-    __shared__ WaveLds wl[NW];
+    __shared__ WaveLds wl[NW * NWC];
+    __shared__ float4 comb[NWC > 1 ? NWC : 1][64];   // NWC > 1: each wave's partial F, count
 
-    const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
+    const int lb = xcd_slab_block();
+    int b, lc;
+    if constexpr (NWC == 1) {
+        b = p.cblk_list ? p.cblk_list[lb] : lb;
+        lc = (b - p.sys[p.cblk_sys[b]].cblk_first) * NW +
+             __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    } else {
+        b = p.lblk_list ? p.lblk_list[lb / p.cpb] : lb / p.cpb;
+        lc = (b - p.sys[p.cblk_sys[b]].cblk_first) * p.cpb + lb % p.cpb;
+    }
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
-    if (S.tiled != 1) return;   // this system's forces come from another kernel
+    if (S.tiled != (NWC > 1 ? 2 : 1)) return;   // this system's forces come from another kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int sub = wid % NWC;   // this wave's share of the neighbour cells
     WaveLds& W = wl[wid];
-    const int lc = (b - S.cblk_first) * NW + wid;
-    if (lc >= S.ncell) return;   // past the system's last cell (no barrier follows)
+    if (lc >= S.ncell) return;   // past the system's last cell (uniform over an NWC block)
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
     if (!KICK || S.active) {
@@ -1436,7 +1255,7 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
             nb_axis(cz, oz, S.nc[2], nbz, iz);
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
-            cntq = p.start[nb + 1] - j0;
+            cntq = (NWC == 1 || lane % NWC == sub) ? p.start[nb + 1] - j0 : 0;
             // kept in LDS for the chunks of this cell (not in registers)
             W.tb_img[lane] = make_float4(S.shf[0][3 * (ox + 1) + ix + 1],
                                             S.shf[1][3 * (oy + 1) + iy + 1],
@@ -1510,10 +1329,14 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
                 const unsigned j = (unsigned)(W.tb_delta[pos] + gc);
                 v = *(const float4*)((const char*)p.lpos + j * 16u);
             };
-            int nq;
-            float4 nv;
-            fetch(lane, nq, nv);
-            const int self0 = off_home + (i0 - h0);   // concatenation index of the chunk's first particle
+            int nq = 0;
+            float4 nv = make_float4(0.f, 0.f, 0.f, 0.f);
+            // (a wave of several per cell may have no candidate at all: then
+            // nothing is fetched, as the staging loop below does not run)
+            if (total > 0) fetch(lane, nq, nv);
+            // concatenation index of the chunk's first particle (only the wave
+            // that walks the home cell, table entry 13, meets the chunk itself)
+            const int self0 = (NWC == 1 || 13 % NWC == sub) ? off_home + (i0 - h0) : 0x3fffffff;
             int g0 = 0;
             double pe = 0.0, w = 0.0;              // constant law
             double ga = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of g, s^6, r^2
@@ -1664,6 +1487,26 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
                     ax += r.x; ay += r.y; az += r.z;
                     nct += __float_as_int(r.w);
                 }
+            }
+            bool owner = act && sl == 0;
+            if constexpr (NWC > 1) {
+                // the waves' partial forces of each particle, added in wave order
+                if (owner) comb[sub][il] = make_float4(ax, ay, az, __int_as_float(nct));
+                __syncthreads();
+                if (owner && sub == 0) {
+                    float4 r = comb[0][il];
+                    ax = r.x; ay = r.y; az = r.z;
+                    nct = __float_as_int(r.w);
+                    for (int w2 = 1; w2 < NWC; ++w2) {
+                        r = comb[w2][il];
+                        ax += r.x; ay += r.y; az += r.z;
+                        nct += __float_as_int(r.w);
+                    }
+                }
+                __syncthreads();   // comb is written again by the next chunk
+                owner = owner && sub == 0;
+            }
+            if (owner) {
                 // the LJ law accumulates F / 24 (exact rescaling in double)
                 constexpr double fscale = (LAW == LAW_LJ) ? 24.0 : 1.0;
                 const double fx = fscale * (double)ax, fy = fscale * (double)ay,
@@ -1685,8 +1528,8 @@ __global__ __launch_bounds__(64 * NW, 6) void k_pair_cells_mixed(StepArgs p)
     // this cell's sums (one record per cell, so the block shape does not
     // change any result)
     double out[4];
-    block_sum4_lds<1>(wacc, reinterpret_cast<double*>(W.st), out);
-    if (lane == 0) {
+    block_sum4_lds<NWC>(wacc, reinterpret_cast<double*>(W.st), out);
+    if (NWC == 1 ? lane == 0 : threadIdx.x == 0) {
         const int c = S.cell_off + lc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial_c[4 * (size_t)c + q] = out[q];
@@ -1731,10 +1574,15 @@ struct SliceGeomTableD {
 
 __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 
-template <int LAW, int KICK>
-__global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(StepArgs p)
+// NWC > 1 (small systems): NWC waves share one home cell, wave sub taking the
+// neighbour cells q with q % NWC == sub; their partial forces are added in
+// wave order before the kick.  The system's flag names the kernel: tiled 1
+// for NWC = 1, 2 for NWC > 1.
+template <int LAW, int KICK, int NWC = 1>
+__global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells_fp64(StepArgs p)
 {
-    constexpr int NW = kCellsPerBlockFp64;
+    static_assert(kCellsPerBlockFp64 == 1 || NWC == 1, "several waves per cell: one cell per block");
+    constexpr int NW = kCellsPerBlockFp64 * NWC;
     __shared__ float4 st[NW][kRowsD];
     __shared__ double sd[NW][kRowsD * 3];      // r_j of each entry; slice and block sums
     __shared__ unsigned char sq[NW][kRowsD];   // table index of the entry's cell
@@ -1742,15 +1590,18 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
     __shared__ int tb_off[NW][32];             // entries 27..31 = INT_MAX (search padding)
     __shared__ double tb_shd[NW][27][3];       // image shift i L of each neighbour cell
     __shared__ double tb_rel[NW][27][3];       // i L - o (o: chunk centre)
+    __shared__ double comb[NWC > 1 ? NWC : 1][64][4];   // NWC > 1: each wave's partial F, count
 
-    const int b = p.cblk_list ? p.cblk_list[xcd_slab_block()] : xcd_slab_block();
+    const int* list = NWC > 1 ? p.lblk_list : p.cblk_list;
+    const int b = list ? list[xcd_slab_block()] : xcd_slab_block();
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
-    if (S.tiled != 1) return;   // this system's forces come from another kernel
+    if (S.tiled != (NWC > 1 ? 2 : 1)) return;   // this system's forces come from another kernel
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lc = (b - S.cblk_first) * kCellsPerBlockFp64 + wid;
-    if (lc >= S.ncell) return;   // past the system's last cell (no barrier follows)
+    const int sub = wid % NWC;   // this wave's share of the neighbour cells
+    const int lc = (b - S.cblk_first) * kCellsPerBlockFp64 + wid / NWC;
+    if (lc >= S.ncell) return;   // past the system's last cell (uniform over the block)
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
     double* sdw = sd[wid];
 
@@ -1779,14 +1630,14 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
             nb_axis(cz, oz, S.nc[2], nbz, iz);
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
-            cntq = p.start[nb + 1] - j0;
+            cntq = (NWC == 1 || lane % NWC == sub) ? p.start[nb + 1] - j0 : 0;
             sx = ix * S.L[0];
             sy = iy * S.L[1];
             sz = iz * S.L[2];
             tb_shd[wid][lane][0] = sx;
             tb_shd[wid][lane][1] = sy;
             tb_shd[wid][lane][2] = sz;
-            img = (ix | iy | iz) != 0;
+            img = (ix | iy | iz) != 0 && cntq > 0;
         }
         // no neighbour cell is an image (cells away from the box faces): every
         // shift is +0.0, and (r_i - r_j) - 0.0 is r_i - r_j bit for bit
@@ -1859,9 +1710,11 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
                 y = p.pos.y[j];
                 z = p.pos.z[j];
             };
-            int nq;
-            double nx, ny, nz;
-            fetch(lane, nq, nx, ny, nz);
+            int nq = 0;
+            double nx = 0.0, ny = 0.0, nz = 0.0;
+            // (a wave of several per cell may have no candidate at all: then
+            // nothing is fetched, as the staging loop below does not run)
+            if (total > 0) fetch(lane, nq, nx, ny, nz);
             int g0 = 0;
             double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, w = 0.0;
             double ga = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of g / 24, s^6, r^2
@@ -2018,6 +1871,31 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
                     fz += sdw[o + 2];
                     nct += (int)sdw[o + 3];
                 }
+            }
+            bool owner = act && sl == 0;
+            if constexpr (NWC > 1) {
+                // the waves' partial forces of each particle, added in wave order
+                if (owner) {
+                    comb[sub][il][0] = fx;
+                    comb[sub][il][1] = fy;
+                    comb[sub][il][2] = fz;
+                    comb[sub][il][3] = (double)nct;
+                }
+                __syncthreads();
+                if (owner && sub == 0) {
+                    fx = comb[0][il][0]; fy = comb[0][il][1]; fz = comb[0][il][2];
+                    nct = (int)comb[0][il][3];
+                    for (int w2 = 1; w2 < NWC; ++w2) {
+                        fx += comb[w2][il][0];
+                        fy += comb[w2][il][1];
+                        fz += comb[w2][il][2];
+                        nct += (int)comb[w2][il][3];
+                    }
+                }
+                __syncthreads();   // comb is written again by the next chunk
+                owner = owner && sub == 0;
+            }
+            if (owner) {
                 if (p.ncount) p.ncount[si] = nct;
                 if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
@@ -2033,8 +1911,8 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64, 4) void k_pair_cells_fp64(
         }
     }
     double out[4];
-    block_sum4_lds<1>(wacc, sdw, out);
-    if (lane == 0) {
+    block_sum4_lds<NWC>(wacc, sdw, out);
+    if (threadIdx.x == 0) {
         const int c = S.cell_off + lc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) p.partial_c[4 * (size_t)c + q] = out[q];

@@ -19,7 +19,7 @@ from lennardjonesium._native import check
 
 LAWS = {"lj": 0, "constant": 1, "none": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
-PAIR_KERNELS = {"particle": 0, "tiled": 1, "split": 2, "lanes": 3}
+PAIR_KERNELS = {"particle": 0, "tiled": 1, "split": 2, "multiwave": 3}
 THERMO_FIELDS = ("time", "kinetic_energy", "potential_energy", "total_energy", "virial",
                  "temperature", "mean_square_displacement")
 
@@ -224,9 +224,9 @@ class Engine:
 
     def set_pair_kernel(self, variant):
         """'tiled' (one wave per home cell, LDS-staged neighbours), 'particle'
-        (one thread per particle), 'lanes' (a workgroup per home cell, 32 lanes
-        per particle, one per neighbour cell; small systems) or 'split' (per
-        system, by cell occupancy)."""
+        (one thread per particle), 'multiwave' (the tiled kernel with four
+        waves per home cell; small systems) or 'split' (per system, by cell
+        occupancy)."""
         check(self.lib.lj_engine_set_pair_kernel(self.h, PAIR_KERNELS[variant]))
 
     def set_pair_split(self, min_occupancy):
@@ -234,7 +234,7 @@ class Engine:
         check(self.lib.lj_engine_set_pair_split(self.h, float(min_occupancy)))
 
     def pair_kernels(self):
-        """The kernel each system uses now ('tiled', 'lanes' or 'particle')."""
+        """The kernel each system uses now ('tiled', 'multiwave' or 'particle')."""
         out = np.zeros(self.nsys, dtype=np.int32)
         check(self.lib.lj_engine_pair_kernels(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         names = {k: n for n, k in PAIR_KERNELS.items()}

@@ -65,7 +65,7 @@ def coexistence_positions():
     return pos, L, F, pe, w, counts
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_coexistence_fp64_cutoff_decisions_and_forces(coexistence_positions, kernel):
     eng = _eng()
     pos, L, F, pe, w, counts = coexistence_positions

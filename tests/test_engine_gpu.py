@@ -110,7 +110,7 @@ def _oracle_counts(pos, L, rc=2.5):
     return np.bincount(idx[:, 0], minlength=n) + np.bincount(idx[:, 1], minlength=n)
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 @pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.3), (500, 1.2, 0.4), (40, 0.05, 0.5),
                                        (4000, 0.3, 1.0), (256000, 0.8442, 0.2),
                                        (32, 1.185, 0.1)])
@@ -129,7 +129,7 @@ def test_production_kernel_cutoff_decisions_bit_exact(n, rho, amp, kernel):
     e.close()
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_mixed_kernel_cutoff_decisions(kernel):
     """FP32 separations may flip only pairs within ~1e-5 of the cutoff."""
     eng = _engine_mod()
@@ -146,7 +146,7 @@ def test_mixed_kernel_cutoff_decisions(kernel):
 
 # ---- forces, PE, virial: FP64 within 1e-12 ------------------------------------------
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 @pytest.mark.parametrize("n,rho,amp", [(864, 0.8, 0.2), (4000, 0.8, 0.3), (500, 1.2, 0.2),
                                        (256000, 0.8442, 0.15), (32, 1.185, 0.1)])
 def test_forces_fp64(n, rho, amp, kernel):
@@ -263,7 +263,7 @@ def test_trajectory_tracks_oracle_and_graphs_are_exact():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_single_particle_drifts_through_the_wrap(precision, kernel):
     """N = 1 in a box of 3 cells a side: its own images lie beyond the cutoff,
     so no pair exists and F = PE = W = 0 exactly; the particle drifts at
@@ -292,7 +292,7 @@ def test_single_particle_drifts_through_the_wrap(precision, kernel):
     np.testing.assert_allclose(rec, ref, rtol=1e-15, atol=0)
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes", "split"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave", "split"])
 def test_batched_replicas_match_single_runs(kernel):
     """One engine for three uneven replicas gives each replica's single-engine
     records bit for bit; 'split' runs the tiled kernel for two replicas and
@@ -368,7 +368,7 @@ def test_set_temperature_and_measure():
 
 # ---- mixed precision: error stated against the FP64 oracle -------------------------------
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 @pytest.mark.parametrize("n,rho", [(4000, 0.8), (256000, 0.8442)])
 def test_mixed_forces_error(n, rho, kernel):
     eng = _engine_mod()
@@ -439,7 +439,7 @@ def test_autotune_leaves_no_trace():
         for k, (p, v) in enumerate(states):
             e.set_state(k, p, v)
     variant, ms = tuned.autotune(4)
-    assert variant in ("tiled", "particle", "lanes", "split") and ms > 0
+    assert variant in ("tiled", "particle", "multiwave", "split") and ms > 0
     ref.set_pair_kernel(variant)
     assert np.array_equal(ref.step(30), tuned.step(30))
 
@@ -562,7 +562,7 @@ def test_frozen_replica_records_read_zero():
     assert np.all(second[:, 1, :] == 0) and np.all(second[:, 0, 0] > first[-1, 0, 0])
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_clumped_cell_chunks_and_stage_splits(kernel):
     """300 particles inside one cell of a 4^3 grid: the tiled kernel walks
     several 64-particle chunks and splits its neighbour stages; results must
@@ -613,7 +613,7 @@ def test_records_independent_of_call_boundaries(graphs):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_chained_steps_leave_the_reference_state(precision, kernel):
     """Inside one call the pair kernel also applies the next step's first half
     kick (forces are then not stored); the last step of a call must leave
@@ -642,7 +642,7 @@ def test_chained_steps_leave_the_reference_state(precision, kernel):
         assert rel.max() < 1e-11
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "particle", "lanes"])
+@pytest.mark.parametrize("kernel", ["tiled", "particle", "multiwave"])
 def test_inactive_replica_inside_chained_graphs(kernel):
     """Step graphs are captured while every replica is active; a replica
     switched off later must stay where it is (its particles still enter the
