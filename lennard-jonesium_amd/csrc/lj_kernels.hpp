@@ -228,13 +228,13 @@ struct StepArgs {
     int skip_tiled;                     // per-particle pair kernel: leave tiled systems alone
     const int* cblk_list;               // tiled pair kernels: cell-space blocks to run (null: all)
     const int* blk_list;                // per-particle pair kernel: blocks to run (null: all)
-    const int* lblk_list;               // several waves per cell (small systems): blocks to run (null: all)
-    int cpb;                            // cells per cell-space block
     int* pending;                       // per system: a step's block sums await the finalize
     const int* cell_xyz;                // per cell: x | y << 10 | z << 20 (cell coordinates)
     int* ncount;                        // optional: in-cutoff neighbours per slot
     int ncell_total;
     int nslot;
+    const int* lblk_list;               // several waves per cell (small systems): blocks to run (null: all)
+    int cpb;                            // cells per cell-space block
 };
 
 // ---------------------------------------------------------------------------
@@ -1214,15 +1214,11 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
     __shared__ float4 comb[NWC > 1 ? NWC : 1][64];   // NWC > 1: each wave's partial F, count
 
     const int lb = xcd_slab_block();
-    int b, lc;
-    if constexpr (NWC == 1) {
+    int b;
+    if constexpr (NWC == 1)
         b = p.cblk_list ? p.cblk_list[lb] : lb;
-        lc = (b - p.sys[p.cblk_sys[b]].cblk_first) * NW +
-             __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    } else {
+    else
         b = p.lblk_list ? p.lblk_list[lb / p.cpb] : lb / p.cpb;
-        lc = (b - p.sys[p.cblk_sys[b]].cblk_first) * p.cpb + lb % p.cpb;
-    }
     const int k = p.cblk_sys[b];
     const SysDev& S = p.sys[k];
     if (S.tiled != (NWC > 1 ? 2 : 1)) return;   // this system's forces come from another kernel
@@ -1230,6 +1226,7 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int sub = wid % NWC;   // this wave's share of the neighbour cells
     WaveLds& W = wl[wid];
+    const int lc = NWC == 1 ? (b - S.cblk_first) * NW + wid : (b - S.cblk_first) * p.cpb + lb % p.cpb;
     if (lc >= S.ncell) return;   // past the system's last cell (uniform over an NWC block)
     double wacc[4] = {0, 0, 0, 0};   // |v|^2, PE, W, |d|^2 of this lane's owned particles
 
@@ -1333,7 +1330,7 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
             float4 nv = make_float4(0.f, 0.f, 0.f, 0.f);
             // (a wave of several per cell may have no candidate at all: then
             // nothing is fetched, as the staging loop below does not run)
-            if (total > 0) fetch(lane, nq, nv);
+            if (NWC == 1 || total > 0) fetch(lane, nq, nv);
             // concatenation index of the chunk's first particle (only the wave
             // that walks the home cell, table entry 13, meets the chunk itself)
             const int self0 = (NWC == 1 || 13 % NWC == sub) ? off_home + (i0 - h0) : 0x3fffffff;
@@ -1637,7 +1634,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
             tb_shd[wid][lane][0] = sx;
             tb_shd[wid][lane][1] = sy;
             tb_shd[wid][lane][2] = sz;
-            img = (ix | iy | iz) != 0 && cntq > 0;
+            img = (ix | iy | iz) != 0 && (NWC == 1 || cntq > 0);
         }
         // no neighbour cell is an image (cells away from the box faces): every
         // shift is +0.0, and (r_i - r_j) - 0.0 is r_i - r_j bit for bit
@@ -1714,7 +1711,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
             double nx = 0.0, ny = 0.0, nz = 0.0;
             // (a wave of several per cell may have no candidate at all: then
             // nothing is fetched, as the staging loop below does not run)
-            if (total > 0) fetch(lane, nq, nx, ny, nz);
+            if (NWC == 1 || total > 0) fetch(lane, nq, nx, ny, nz);
             int g0 = 0;
             double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, w = 0.0;
             double ga = 0.0, s6a = 0.0, r2a = 0.0;   // LJ: sums of g / 24, s^6, r^2
