@@ -226,6 +226,24 @@ int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, doubl
 /* 0 disables HIP graph capture of the step loop (launches each kernel). */
 int lj_engine_set_graphs(lj_engine* e, int enabled);
 
+/* Resident cells (default on): a step call of at least 4 steps on an engine
+ * whose systems all use the one-wave tiled pair kernel (LJ_PAIR_CELL_TILED)
+ * builds its cell lists once, keeps every cell in a fixed block of slots and,
+ * in each later step, moves only the particles that changed cell (in place of
+ * the key pass, scan and sort); the call's last step packs the state back.
+ * Records and state are the same bit for bit as with it off.  A call in which
+ * a cell would exceed its slots (56 members) or receive more than 8 particles
+ * in one step is redone from its starting state with the full cell build
+ * (same results, more time).  The first such call allocates the resident
+ * buffers (about 190 bytes per cell slot, 56 slots per cell).  No counterpart
+ * in the reference, which rebuilds its cell lists every step
+ * (engine/particle_pair_filter.cpp:141-156). */
+int lj_engine_set_resident_cells(lj_engine* e, int enabled);
+
+/* Calls run on resident cells since creation, and how many of them were
+ * redone with the full cell build. */
+int lj_engine_resident_stats(const lj_engine* e, long long* calls, long long* fallbacks);
+
 /* Pair-kernel variant: LJ_PAIR_CELL_TILED (default) runs one wave64 per home
  * cell with the neighbour cells staged in LDS; LJ_PAIR_PER_PARTICLE runs one
  * thread per particle over the 27 neighbour cells; LJ_PAIR_BY_OCCUPANCY picks

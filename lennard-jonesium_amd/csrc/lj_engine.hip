@@ -177,6 +177,24 @@ struct lj_engine {
     int graph_cap = -1;               // record capacity baked into the graph
     const double* graph_rec = nullptr;   // record buffer baked into the graph
 
+    // Resident cells (chained steps without a cell build; lj_kernels.hpp
+    // section 4b): buffers allocated on first use, graphs of resident steps.
+    bool res_enabled = true;
+    bool res_ready = false;
+    int res_nslot = 0;                 // kResCap slots per real cell
+    int res_nfix = 0;                  // fix-up blocks (the listed cells, one wave each)
+    double* r_pos[2] = {nullptr, nullptr};
+    double* r_vel[2] = {nullptr, nullptr};
+    double* r_disp[2] = {nullptr, nullptr};
+    int* r_id[2] = {nullptr, nullptr};
+    float4* r_lpos[2] = {nullptr, nullptr};
+    int *r_base = nullptr, *r_count = nullptr, *r_end = nullptr, *r_dirty = nullptr;
+    int *r_narr = nullptr, *r_flag = nullptr, *d_cell_sys = nullptr;
+    ljk::ResMover* r_arr = nullptr;
+    double* r_ck = nullptr;            // a call's starting forces, time, PE, W (for a fallback)
+    hipGraphExec_t rgraphs[2][5] = {}; // resident steps: 2^(l+1) steps from parity s
+    long long res_calls = 0, res_fallbacks = 0;
+
     bool profiling = false;
     std::vector<KernelTimer> timers;
     // Profiling events: pairs (2k, 2k+1) bracket the k-th pending launch;
@@ -227,7 +245,54 @@ struct lj_engine {
         p.ncount = d_ncount;
         p.ncell_total = ncell_total;
         p.nslot = nslot;
+        p.cend = d_start ? d_start + 1 : nullptr;
+        p.lpos_n = d_lpos;
+        p.rc_base = r_base;
+        p.rc_count = r_count;
+        p.rc_end = r_end;
+        p.rc_dirty = r_dirty;
+        p.rc_narr = r_narr;
+        p.rc_arr = r_arr;
+        p.rc_flag = r_flag;
+        p.cell_sys = d_cell_sys;
+        p.rc_par = 0;
         return p;
+    }
+
+    // A resident step of parity r (record parts and pending flags): positions
+    // and cell-local coordinates from set r, the next ones into set r ^ 1;
+    // velocities, displacements and ids in place.  next: the fix-up's view,
+    // positions and coordinates of set r ^ 1.
+    ljk::StepArgs args_res(int r, bool next = false) const
+    {
+        ljk::StepArgs p = args(cur);
+        auto planes = [&](double* base) {
+            return ljk::Planes{base, base + res_nslot, base + 2 * (size_t)res_nslot};
+        };
+        const int a = next ? r ^ 1 : r;
+        p.pos = planes(r_pos[a]);
+        p.pos_n = planes(r_pos[r ^ 1]);
+        p.vel = p.vel_n = planes(r_vel[0]);
+        p.disp = p.disp_n = planes(r_disp[0]);
+        p.id = p.id_n = r_id[0];
+        p.lpos = r_lpos[a];
+        p.lpos_n = r_lpos[r ^ 1];
+        p.start = const_cast<int*>(r_base);
+        p.cend = r_end;
+        p.pending = d_pending + (size_t)r * nsys;
+        p.nslot = res_nslot;
+        p.rc_par = r;
+        return p;
+    }
+
+    // fargs(0) with the record parts and pending flags of parity par (the
+    // packed path uses parity 0).
+    ljk::FinalArgs fargs_par(int par) const
+    {
+        ljk::FinalArgs f = fargs(0);
+        f.partial2 = d_partial2 + (size_t)par * nsys * ljk::kMaxParts * 4;
+        f.pending = d_pending + (size_t)par * nsys;
+        return f;
     }
 
     ljk::FinalArgs fargs(int mode) const
@@ -625,11 +690,12 @@ int capture_steps(lj_engine* e, int from, int nsteps, hipGraphExec_t* out)
 
 void drop_graphs(lj_engine* e)
 {
-    for (auto& set : e->graphs)
-        for (hipGraphExec_t& g : set) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto* sets : {&e->graphs, &e->rgraphs})
+        for (auto& set : *sets)
+            for (hipGraphExec_t& g : set) {
+                if (g) (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 int build_graph(lj_engine* e)
@@ -756,6 +822,9 @@ int check_sys(const lj_engine* e, int sys)
 }
 
 int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records);
+bool res_eligible(const lj_engine* e, int nsteps);
+int res_ensure(lj_engine* e);
+int build_res_graphs(lj_engine* e);
 
 // Ensures slots are in cell order without moving particles (needed before
 // any traversal when the state was just loaded).
@@ -945,8 +1014,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         (rc = dev_alloc(&e->d_lblk_list, (size_t)cblk)) ||
         (rc = dev_alloc(&e->d_time, nsys)) || (rc = dev_alloc(&e->d_pe, nsys)) ||
         (rc = dev_alloc(&e->d_vir, nsys)) || (rc = dev_alloc(&e->d_step, nsys)) ||
-        (rc = dev_alloc(&e->d_pending, nsys)) ||
-        (rc = dev_alloc(&e->d_partial2, (size_t)nsys * ljk::kMaxParts * 4)) ||
+        (rc = dev_alloc(&e->d_pending, 2 * (size_t)nsys)) ||
+        (rc = dev_alloc(&e->d_partial2, 2 * (size_t)nsys * ljk::kMaxParts * 4)) ||
         (rc = dev_alloc(&e->d_measure, 7 * (size_t)nsys)))
         return cleanup_fail(rc);
     for (int b = 0; b < 2; ++b) {
@@ -990,7 +1059,7 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
               h2d(e->d_time, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_pe, zeros.data(), nsys * sizeof(double)) &&
               h2d(e->d_vir, zeros.data(), nsys * sizeof(double)) &&
-              h2d(e->d_pending, std::vector<int>(nsys, 0).data(), nsys * sizeof(int));
+              h2d(e->d_pending, std::vector<int>(2 * nsys, 0).data(), 2 * nsys * sizeof(int));
     for (int b = 0; b < 2 && ok; ++b)
         ok = h2d(e->d_pos[b], zeros.data(), 3 * ns * sizeof(double)) &&
              h2d(e->d_vel[b], zeros.data(), 3 * ns * sizeof(double)) &&
@@ -1032,10 +1101,15 @@ int lj_engine_destroy(lj_engine* e)
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int b = 0; b < 2; ++b) {
-        void* more[] = {e->d_pos[b], e->d_vel[b], e->d_disp[b], e->d_id[b]};
+        void* more[] = {e->d_pos[b], e->d_vel[b], e->d_disp[b], e->d_id[b],
+                        e->r_pos[b], e->r_vel[b], e->r_disp[b], e->r_id[b], e->r_lpos[b]};
         for (void* m : more)
             if (m) (void)hipFree(m);
     }
+    void* res[] = {e->r_base, e->r_count, e->r_end, e->r_dirty, e->r_narr,
+                   e->r_arr, e->r_flag, e->d_cell_sys, e->r_ck};
+    for (void* b : res)
+        if (b) (void)hipFree(b);
     for (hipEvent_t ev : e->prof_ev) (void)hipEventDestroy(ev);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
@@ -1157,12 +1231,307 @@ int lj_engine_reserve(lj_engine* e, int max_steps)
     if (max_steps < 0) return fail(LJ_E_INVALID, "max_steps < 0");
     if ((rc = ensure_records(e, max_steps))) return rc;
     if (e->use_graphs && !e->profiling && e->has_cells && (rc = build_graph(e))) return rc;
+    if (res_eligible(e, std::max(max_steps, 4)) && (rc = res_ensure(e))) return rc;
+    if (e->res_ready && e->use_graphs && !e->profiling && res_eligible(e, 4) &&
+        (rc = build_res_graphs(e)))
+        return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return LJ_OK;
 }
 }  // extern "C"
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Resident cells (lj_kernels.hpp section 4b).  A call of n >= kResMinSteps
+// steps on an engine whose systems all use the one-wave tiled kernel runs
+//   step 1:      key pass -> k_res_enter (into the resident slots) -> pair -> drift -> fix-up
+//   steps 2..n-1: pair -> drift -> fix-up (replayed from graphs of resident steps)
+//   step n:      scan of the resident counts -> k_res_leave (packed set) -> pair
+// The pair kernels see the same cells in the same order as on the packed
+// path, so records and state are the same bit for bit.  The packed set the
+// call started from is not written, so when a capacity was exceeded the call
+// is redone on the packed path from there (the forces, time, PE and W are
+// checkpointed at its start).
+// ---------------------------------------------------------------------------
+constexpr int kResMinSteps = 4;
+
+bool res_eligible(const lj_engine* e, int nsteps)
+{
+    if (!e->res_enabled || !e->has_cells || nsteps < kResMinSteps) return false;
+    if (e->n_tiled != e->nsys) return false;   // one-wave tiled kernel for every system
+    long long slots = 0;
+    for (const SysDev& s : e->sys) {
+        if (!s.active || s.law == ljk::LAW_NONE) return false;
+        slots += (long long)s.ncell * ljk::kResCap;
+    }
+    // resident slots: within the slot limit and at most 16 per particle
+    return slots <= kMaxSlots && slots <= 16LL * e->nslot + (1 << 20);
+}
+
+// Allocates and initialises the resident buffers (once per engine).
+int res_ensure(lj_engine* e)
+{
+    if (e->res_ready) return LJ_OK;
+    const int nc = e->ncell_total;
+    std::vector<int> base(nc, 0), count(nc, 0), csys(nc, 0);
+    std::vector<int> count_init(nc, 0);
+    HIP_TRY(hipMemcpy(count_init.data(), e->d_count_init, nc * sizeof(int), hipMemcpyDeviceToHost));
+    long long off = 0;
+    for (int k = 0; k < e->nsys; ++k) {
+        const SysDev& s = e->sys[k];
+        for (int lc = 0; lc <= s.ncell; ++lc) {
+            const int c = s.cell_off + lc;
+            csys[c] = k;
+            base[c] = (int)(off + (long long)std::min(lc, s.ncell) * ljk::kResCap);
+            if (lc == s.ncell) count[c] = count_init[c];   // tail cell: padding slots
+        }
+        off += (long long)s.ncell * ljk::kResCap;
+    }
+    const size_t ns = (size_t)off;
+    int rc;
+    for (int b = 0; b < 2; ++b)
+        if ((rc = dev_alloc(&e->r_pos[b], 3 * ns)) ||
+            (e->precision == LJ_PRECISION_MIXED && (rc = dev_alloc(&e->r_lpos[b], ns))))
+            return rc;
+    if ((rc = dev_alloc(&e->r_vel[0], 3 * ns)) || (rc = dev_alloc(&e->r_disp[0], 3 * ns)) ||
+        (rc = dev_alloc(&e->r_id[0], ns)))
+        return rc;
+    if ((rc = dev_alloc(&e->r_base, nc)) || (rc = dev_alloc(&e->r_count, nc)) ||
+        (rc = dev_alloc(&e->r_end, nc)) || (rc = dev_alloc(&e->r_dirty, nc)) ||
+        (rc = dev_alloc(&e->r_narr, nc)) || (rc = dev_alloc(&e->r_arr, (size_t)nc * ljk::kResArr)) ||
+        (rc = dev_alloc(&e->r_flag, 1)) || (rc = dev_alloc(&e->d_cell_sys, nc)) ||
+        (rc = dev_alloc(&e->r_ck, 3 * (size_t)e->nslot + 3 * (size_t)e->nsys)))
+        return rc;
+    // every slot starts as zeros (finite values wherever a kernel may read)
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMemset(e->r_pos[b], 0, 3 * ns * sizeof(double)));
+        if (e->r_lpos[b]) HIP_TRY(hipMemset(e->r_lpos[b], 0, ns * sizeof(float4)));
+    }
+    HIP_TRY(hipMemset(e->r_vel[0], 0, 3 * ns * sizeof(double)));
+    HIP_TRY(hipMemset(e->r_disp[0], 0, 3 * ns * sizeof(double)));
+    HIP_TRY(hipMemset(e->r_id[0], 0, ns * sizeof(int)));
+    HIP_TRY(hipMemcpy(e->r_base, base.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->r_end, base.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->r_count, count.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_cell_sys, csys.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(e->r_dirty, 0, nc * sizeof(int)));
+    HIP_TRY(hipMemset(e->r_narr, 0, nc * sizeof(int)));
+    HIP_TRY(hipMemset(e->r_arr, 0, (size_t)nc * ljk::kResArr * sizeof(ljk::ResMover)));
+    HIP_TRY(hipMemset(e->r_flag, 0, sizeof(int)));
+    e->res_nslot = (int)ns;
+    // fix-up waves of kResFixCells cells
+    const int waves = (nc + ljk::kResFixCells - 1) / ljk::kResFixCells;
+    e->res_nfix = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    e->res_ready = true;
+    return LJ_OK;
+}
+
+template <int LAW>
+void launch_res_pair(lj_engine* e, const ljk::StepArgs& p)
+{
+    const dim3 grid(e->ncblk);
+    if (e->precision == LJ_PRECISION_MIXED) {
+        if (e->cells_per_block == 2)
+            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 2, 1, true>, grid, dim3(128), p);
+        else if (e->cells_per_block == 8)
+            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 8, 1, true>, grid, dim3(512), p);
+        else
+            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 4, 1, true>, grid, dim3(256), p);
+    } else {
+        launch(e, ljk::k_pair_cells_fp64<LAW, 2, 1, true>, grid, dim3(64 * ljk::kCellsPerBlockFp64), p);
+    }
+}
+
+// One resident step of parity r: the pair kernel with the drift and the
+// fix-up (first: no previous resident step, whose record the fix-up would
+// finish).
+int enqueue_res_step(lj_engine* e, int r, bool first)
+{
+    const ljk::StepArgs p = e->args_res(r);
+    const bool mixed = e->precision == LJ_PRECISION_MIXED;
+    e->last_pair_force = true;
+    int rc = timed(e, 5, "pair", [&] {
+        if (e->sys[0].law == ljk::LAW_LJ) launch_res_pair<ljk::LAW_LJ>(e, p);
+        else launch_res_pair<ljk::LAW_CONSTANT>(e, p);
+    });
+    if (rc) return rc;
+    const ljk::StepArgs q = e->args_res(r, true);
+    const ljk::FinalArgs f1 = e->fargs_par(r), f2 = e->fargs_par(r ^ 1);
+    const int nred = first ? 0 : e->nsys;
+    const int nfin = e->nsys * f1.nf;
+    return timed(e, 9, "fixup", [&] {
+        const dim3 grid(nred + nfin + e->res_nfix);
+        if (mixed) launch(e, ljk::k_res_fixup<true>, grid, dim3(kBlock), q, f1, f2, nred, nfin);
+        else launch(e, ljk::k_res_fixup<false>, grid, dim3(kBlock), q, f1, f2, nred, nfin);
+    });
+}
+
+// Captures nsteps resident steps starting at parity `from` (none the first).
+int capture_res_steps(lj_engine* e, int from, int nsteps, hipGraphExec_t* out)
+{
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = LJ_OK;
+    int r = from;
+    for (int s = 0; s < nsteps && !rc; ++s, r ^= 1) rc = enqueue_res_step(e, r, false);
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return fail(LJ_E_HIP, std::string("hipStreamEndCapture: ") +
+                                                    hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(LJ_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    return LJ_OK;
+}
+
+int build_res_graphs(lj_engine* e)
+{
+    // the packed graphs' checks decide when both kinds are stale (records)
+    int rc = build_graph(e);
+    if (rc) return rc;
+    if (e->rgraphs[1][kGraphLevels - 1]) return LJ_OK;
+    for (int from = 0; from < 2 && !rc; ++from)
+        for (int l = 0; l < kGraphLevels && !rc; ++l)
+            rc = capture_res_steps(e, from, 2 << l, &e->rgraphs[from][l]);
+    if (rc) drop_graphs(e);
+    return rc;
+}
+
+// The step loop on resident cells (see above).  Returns 1 (with nothing to
+// report) when a capacity was exceeded and the caller must redo the call on
+// the packed path; the engine is then back at the call's starting state.
+int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
+{
+    int rc;
+    const int cur0 = e->cur;
+    const bool sorted0 = e->sorted;
+    const bool mixed = e->precision == LJ_PRECISION_MIXED;
+    const size_t ns = e->nslot;
+    const int nsys = e->nsys;
+    e->res_calls++;
+    if (graphs && (rc = build_res_graphs(e))) return rc;
+    // checkpoint of what the call overwrites outside the starting buffer set
+    HIP_TRY(hipMemcpyAsync(e->r_ck, e->d_force, 3 * ns * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns, e->d_time, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns + nsys, e->d_pe, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns + 2 * nsys, e->d_vir, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(e->r_flag, 0, sizeof(int), e->stream));
+    const ljk::FinalArgs f = e->fargs(0);
+    const long long nrec = (long long)nsteps * nsys * 7;
+    const long long nthr = std::max<long long>(nrec, nsys);
+    hipLaunchKernelGGL(ljk::k_begin_call,
+                       dim3((unsigned)std::min<long long>(4096, (nthr + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, e->stream, f.records, nrec, e->d_step, nsys);
+    // step 1: key pass of the integrated positions, then into the resident slots
+    {
+        const ljk::StepArgs p = e->args(cur0);
+        if ((rc = timed(e, 0, "integrate_key", [&] {
+                 launch(e, ljk::k_integrate_key<2, true>, dim3(e->nblk), dim3(kBlock), p, f, 0);
+             })))
+            return rc;
+        ljk::StepArgs q = p;
+        const ljk::StepArgs r0 = e->args_res(0);
+        q.pos_n = r0.pos;
+        q.vel_n = r0.vel;
+        q.disp_n = r0.disp;
+        q.id_n = r0.id;
+        q.lpos_n = r0.lpos;
+        q.rc_par = 0;
+        const dim3 grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
+        if ((rc = timed(e, 10, "res_enter", [&] {
+                 if (mixed) launch(e, ljk::k_res_enter<true, 1>, grid, dim3(kBlock), q);
+                 else launch(e, ljk::k_res_enter<false, 1>, grid, dim3(kBlock), q);
+             })))
+            return rc;
+    }
+    int r = 0;
+    if ((rc = enqueue_res_step(e, r, true))) return rc;
+    r ^= 1;
+    // steps 2 .. n - 1
+    int done = 1;
+    const int middle_end = nsteps - 1;
+    if (graphs) {
+        for (int l = kGraphLevels - 1; l >= 0; --l) {
+            const int len = 2 << l;
+            while (middle_end - done >= len) {
+                HIP_TRY(hipGraphLaunch(e->rgraphs[r][l], e->stream));
+                done += len;   // even: r unchanged
+            }
+        }
+    }
+    for (; done < middle_end; ++done, r ^= 1)
+        if ((rc = enqueue_res_step(e, r, false))) return rc;
+    // step n: pack the resident slots into the other packed set (the scan's
+    // extra blocks finish step n - 1's record), then the pair kernel as usual
+    {
+        ljk::StepArgs p = e->args(cur0 ^ 1);
+        p.count = e->r_count;
+        p.count_init = e->r_count;   // the two-level scan's count reset leaves them as they are
+        const ljk::FinalArgs fl = e->fargs_par(r ^ 1);
+        if (!e->two_level_scan()) {
+            if ((rc = timed(e, 1, "scan", [&] {
+                     launch(e, ljk::k_scan_one, dim3(1 + nsys), dim3(ljk::kScanOneThreads), p, fl);
+                 })))
+                return rc;
+        } else {
+            if ((rc = timed(e, 1, "scan_tiles", [&] {
+                     launch(e, ljk::k_scan_tiles, dim3(e->ntile), dim3(kBlock), p);
+                 })))
+                return rc;
+            if ((rc = timed(e, 2, "scan_apply", [&] {
+                     launch(e, ljk::k_scan_apply, dim3(e->ntile + nsys), dim3(kBlock), p, fl);
+                 })))
+                return rc;
+        }
+        ljk::StepArgs q = e->args_res(r);   // the state of parity r
+        const ljk::StepArgs pk = e->args(cur0 ^ 1);
+        q.pos_n = pk.pos;
+        q.vel_n = pk.vel;
+        q.disp_n = pk.disp;
+        q.id_n = pk.id;
+        q.lpos_n = e->d_lpos;
+        q.start = e->d_start;
+        const dim3 grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
+        if ((rc = timed(e, 11, "res_leave", [&] {
+                 if (mixed) launch(e, ljk::k_res_leave<true>, grid, dim3(kBlock), q);
+                 else launch(e, ljk::k_res_leave<false>, grid, dim3(kBlock), q);
+             })))
+            return rc;
+        e->cur = cur0 ^ 1;
+        e->sorted = true;
+        e->prekicked = false;
+        if ((rc = enqueue_pair(e, 1, false))) return rc;
+    }
+    if ((rc = enqueue_finalize(e, 0))) return rc;
+    HIP_TRY(hipGetLastError());
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, e->r_flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    if (records)
+        HIP_TRY(hipMemcpyAsync(records, f.records, (size_t)nsteps * nsys * sizeof(lj_thermo),
+                               hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (!flag) return LJ_OK;
+    // a capacity was exceeded: back to the call's starting state
+    e->res_fallbacks++;
+    e->cur = cur0;
+    e->sorted = sorted0;
+    e->prekicked = false;
+    HIP_TRY(hipMemcpyAsync(e->d_force, e->r_ck, 3 * ns * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_time, e->r_ck + 3 * ns, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_pe, e->r_ck + 3 * ns + nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_vir, e->r_ck + 3 * ns + 2 * nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(e->d_pending, 0, 2 * nsys * sizeof(int), e->stream));
+    HIP_TRY(hipMemsetAsync(e->r_dirty, 0, e->ncell_total * sizeof(int), e->stream));
+    HIP_TRY(hipMemsetAsync(e->r_narr, 0, e->ncell_total * sizeof(int), e->stream));
+    HIP_TRY(hipMemsetAsync(e->r_flag, 0, sizeof(int), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 1;
+}
 
 // The step loop shared by the step entry points: nsteps steps of every
 // system, records into the engine's current record buffer (fargs), replayed
@@ -1171,6 +1540,12 @@ namespace {
 int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
 {
     int rc;
+    // resident cells: allocating callers always, the others once allocated
+    if (res_eligible(e, nsteps) && (e->res_ready || !e->rec_buf)) {
+        if ((rc = res_ensure(e))) return rc;
+        rc = run_steps_res(e, nsteps, graphs, records);
+        if (rc != 1) return rc;
+    }
     const ljk::FinalArgs f = e->fargs(0);
     const long long nrec = (long long)nsteps * e->nsys * 7;
     const long long nthr = std::max<long long>(nrec, e->nsys);
@@ -1436,6 +1811,22 @@ int lj_engine_set_graphs(lj_engine* e, int enabled)
     int rc = check_engine(e);
     if (rc) return rc;
     e->use_graphs = enabled != 0;
+    return LJ_OK;
+}
+
+int lj_engine_set_resident_cells(lj_engine* e, int enabled)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    e->res_enabled = enabled != 0;
+    return LJ_OK;
+}
+
+int lj_engine_resident_stats(const lj_engine* e, long long* calls, long long* fallbacks)
+{
+    if (!e) return fail(LJ_E_INVALID, "engine is null");
+    if (calls) *calls = e->res_calls;
+    if (fallbacks) *fallbacks = e->res_fallbacks;
     return LJ_OK;
 }
 

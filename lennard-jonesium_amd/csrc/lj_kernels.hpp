@@ -235,6 +235,36 @@ struct StepArgs {
     int nslot;
     const int* lblk_list;               // several waves per cell (small systems): blocks to run (null: all)
     int cpb;                            // cells per cell-space block
+    // End of each cell's slots: start + 1 in the packed layout (start[c + 1]),
+    // its own array for resident cells (section 4b).
+    const int* cend;
+    // Resident cells (section 4b): every real cell owns kResCap slots from
+    // rc_base[c]; a step's pair kernel moves each particle into the next
+    // buffer set itself and hands the few that change cell to the fix-up.
+    float4* lpos_n;                     // next buffers' cell-local float coordinates
+    const int* rc_base;                 // first slot of each cell
+    int* rc_count;                      // members per cell (tail cells: their padding count)
+    int* rc_end;                        // rc_base + rc_count
+    int* rc_dirty;                      // per cell: to be fixed up this step
+    int* rc_narr;                       // per cell: arrivals this step
+    struct ResMover* rc_arr;            // [cell][kResArr]: the arrivals' integrated states
+    int* rc_flag;                       // nonzero: a capacity was exceeded (the call is redone packed)
+    const int* cell_sys;                // system of each cell
+    int rc_par;                         // this step's parity (record parts and pending flags)
+};
+
+// Resident cells: slots per cell (a cell with more members makes the call
+// fall back to the packed layout) and arrivals per cell and step.
+constexpr int kResCap = 56;
+constexpr int kResArr = 8;
+constexpr int kResFixCells = 2;   // cells per fix-up wave
+static_assert(kResCap + kResArr <= 64, "a cell's members and arrivals share one wave");
+
+// A particle that left its cell during a resident step: its integrated state
+// and its new cell.
+struct ResMover {
+    double x, y, z, vx, vy, vz, dx, dy, dz;
+    int id, cell;
 };
 
 // ---------------------------------------------------------------------------
@@ -269,6 +299,16 @@ __device__ int parts_of(const FinalArgs& f, int k);
 // 1. kick + drift + wrap + cell key + cell count
 //    engine/integrator.cpp:69-78 and engine/particle_pair_filter.cpp:141-154
 // ---------------------------------------------------------------------------
+// r - floor(r / L) L (engine/boundary_condition.cpp:45-48).  Inside the box,
+// 0 < r < L (1 - 2^-50), the rounded quotient is below 1, the floor is 0 and
+// the result is r itself bit for bit, so the division is skipped there (a
+// wave divides only when one of its particles left the box).
+__device__ __forceinline__ double wrap_axis(double r, double L)
+{
+    if (r > 0.0 && r < L - L * 0x1p-50) return r;
+    return r - floor(r / L) * L;
+}
+
 // First half kick, drift and periodic wrap of one particle (velocity Verlet,
 // engine/integrator.cpp:69-78, engine/boundary_condition.cpp:45-48):
 // v += (1/2) F dt with the stored force (zero before the first calculation),
@@ -290,9 +330,9 @@ __device__ __forceinline__ void advance(const SysDev& S, double& x, double& y, d
     ix = vx * dt; iy = vy * dt; iz = vz * dt;
     x = x + ix; y = y + iy; z = z + iz;
     if (S.periodic) {
-        x = x - floor(x / S.L[0]) * S.L[0];
-        y = y - floor(y / S.L[1]) * S.L[1];
-        z = z - floor(z / S.L[2]) * S.L[2];
+        x = wrap_axis(x, S.L[0]);
+        y = wrap_axis(y, S.L[1]);
+        z = wrap_axis(z, S.L[2]);
     }
 }
 
@@ -783,6 +823,204 @@ __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 }
 
 // ---------------------------------------------------------------------------
+// 4b. resident cells: chained steps without a cell build.
+//
+//   Every real cell owns kResCap slots from rc_base[c] in the resident
+//   buffers; its members occupy the first rc_count[c] of them in
+//   particle-id order, exactly the order of the packed layout's cell lists.
+//   A resident step is two kernels: the tiled pair kernel (RES), whose owner
+//   lanes also drift and wrap their particles and hand the few that left
+//   their cell to the fix-up as mover records (res_kick_drift), and the
+//   fix-up, which rebuilds only the cells that lost or gained members (again
+//   in id order).  Positions and cell-local coordinates have two buffer sets
+//   (neighbours read the current one while owners write the next);
+//   velocities, displacements and ids are updated in place.  The pair kernels therefore see the same candidates in the
+//   same order as after a full cell build, and results are the same bit for
+//   bit.  k_res_enter moves a call's first cell build into the resident
+//   slots; k_res_leave packs the state back (after a scan of rc_count) for
+//   the call's last step.  Capacities exceeded anywhere (a cell beyond
+//   kResCap, more than kResArr arrivals in a step) set rc_flag; every kernel
+//   stays within its buffers regardless, and the host then redoes the call
+//   on the packed path from the state the call started from.
+// ---------------------------------------------------------------------------
+
+// The first cell build of a call into the resident slots: one wave per cell takes
+// the key pass's member records, ranks them by id and moves each (with the
+// integration of k_sort_cells) to rc_base[c] + rank.  Resets the key pass's
+// counts.
+template <bool MIXED_, int INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_res_enter(StepArgs p)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (c >= p.ncell_total) return;
+    const SysDev& S = p.sys[p.cell_sys[c]];
+    const int lc = c - S.cell_off;
+    if (lc >= S.ncell) return;   // tail cell: rc_count keeps its padding count
+    const int n = __builtin_amdgcn_readfirstlane(p.count[c]);
+    const int base = p.rc_base[c];
+    int kept = n;
+    if (n > min(p.cell_cap, kResCap)) {   // beyond the member records or the resident slots
+        if (lane == 0) atomicOr(p.rc_flag, 1);
+        kept = 0;
+    }
+    if (kept > 0) {
+        const bool act = lane < kept;
+        const int s = p.members[(size_t)c * p.cell_cap + (act ? lane : 0)];
+        const int me = act ? p.id[s] : 0x7fffffff;
+        int rank = 0;
+        for (int j = 0; j < kept; ++j) rank += (__builtin_amdgcn_readlane(me, j) < me) ? 1 : 0;
+        if (act) {
+            double x = p.pos.x[s], y = p.pos.y[s], z = p.pos.z[s];
+            double vx = p.vel.x[s], vy = p.vel.y[s], vz = p.vel.z[s];
+            double dx = p.disp.x[s], dy = p.disp.y[s], dz = p.disp.z[s];
+            if (INTEGRATE && S.active) {
+                double fx = 0., fy = 0., fz = 0.;
+                if (INTEGRATE == 1) { fx = p.force.x[s]; fy = p.force.y[s]; fz = p.force.z[s]; }
+                double ix, iy, iz;
+                advance<INTEGRATE == 1>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+                dx = dx + ix; dy = dy + iy; dz = dz + iz;
+            }
+            const int dst = base + rank;
+            p.pos_n.x[dst] = x; p.pos_n.y[dst] = y; p.pos_n.z[dst] = z;
+            p.vel_n.x[dst] = vx; p.vel_n.y[dst] = vy; p.vel_n.z[dst] = vz;
+            p.disp_n.x[dst] = dx; p.disp_n.y[dst] = dy; p.disp_n.z[dst] = dz;
+            p.id_n[dst] = me;
+            if (MIXED_) {
+                int cx, cy, cz;
+                cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+                p.lpos_n[dst] = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                                            (float)(z - cz * S.a[2]), 0.0f);
+            }
+        }
+    }
+    if (lane == 0) {
+        p.rc_count[c] = kept;
+        p.rc_end[c] = base + kept;
+        p.count[c] = 0;   // ready for the next key pass
+    }
+}
+
+// Marks cell c for this step's fix-up (a plain store: every writer stores 1;
+// no counter that all movers would share).
+__device__ __forceinline__ void res_mark(const StepArgs& p, int c) { p.rc_dirty[c] = 1; }
+
+// Rebuilds one marked cell c of the resident buffers of the next step (p.pos
+// and p.lpos: the sets the pair kernel wrote): its remaining members
+// (vacated slots hold id -1) and its arrivals, in id order from rc_base[c];
+// clears the cell's step marks.  One wave.
+template <bool MIXED_>
+__device__ __forceinline__ void res_fix_cell(const StepArgs& p, int c, int k, int base, int n0,
+                                             int na, int lane)
+{
+    const SysDev& S = p.sys[k];
+    const int tot = n0 + na;   // <= kResCap + kResArr <= 64
+    double x = 0., y = 0., z = 0., vx = 0., vy = 0., vz = 0., dx = 0., dy = 0., dz = 0.;
+    int me = 0x7fffffff;       // no particle
+    if (lane < n0) {
+        const int s = base + lane;
+        me = p.id[s];
+        x = p.pos.x[s]; y = p.pos.y[s]; z = p.pos.z[s];
+        vx = p.vel.x[s]; vy = p.vel.y[s]; vz = p.vel.z[s];
+        dx = p.disp.x[s]; dy = p.disp.y[s]; dz = p.disp.z[s];
+    } else if (lane < tot) {
+        const ResMover& mv = p.rc_arr[(size_t)c * kResArr + (lane - n0)];
+        me = mv.id;
+        x = mv.x; y = mv.y; z = mv.z;
+        vx = mv.vx; vy = mv.vy; vz = mv.vz;
+        dx = mv.dx; dy = mv.dy; dz = mv.dz;
+    }
+    if (me < 0) me = 0x7fffffff;   // a vacated slot
+    const bool valid = me != 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < tot; ++j) rank += (__builtin_amdgcn_readlane(me, j) < me) ? 1 : 0;
+    const int nn = __popcll(__ballot(valid));
+    const int keep = min(nn, kResCap);
+    // every lane's loads are consumed above, before any slot is written
+    if (valid && rank < kResCap) {
+        const int dst = base + rank;
+        p.pos.x[dst] = x; p.pos.y[dst] = y; p.pos.z[dst] = z;
+        p.vel.x[dst] = vx; p.vel.y[dst] = vy; p.vel.z[dst] = vz;
+        p.disp.x[dst] = dx; p.disp.y[dst] = dy; p.disp.z[dst] = dz;
+        p.id[dst] = me;
+        if (MIXED_) {
+            int cx, cy, cz;
+            cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
+            p.lpos[dst] = make_float4((float)(x - cx * S.a[0]), (float)(y - cy * S.a[1]),
+                                      (float)(z - cz * S.a[2]), 0.0f);
+        }
+    }
+    if (lane == 0) {
+        if (nn > kResCap) atomicOr(p.rc_flag, 1);
+        p.rc_count[c] = keep;
+        p.rc_end[c] = base + keep;
+        p.rc_narr[c] = 0;
+        p.rc_dirty[c] = 0;
+    }
+}
+
+// The fix-up after a resident step's pair kernel.  Blocks [0, nred): the second
+// level of the previous step's record (f2: its parity's parts and pending
+// flags); [nred, nred + nfin): the first level of this step's record (f1);
+// the rest: waves of kResFixCells consecutive cells, each fixing its marked
+// cells one after another.
+template <bool MIXED_>
+__global__ __launch_bounds__(kBlock) void k_res_fixup(StepArgs p, FinalArgs f1, FinalArgs f2,
+                                                      int nred, int nfin)
+{
+    int b = blockIdx.x;
+    if (b < nred) {
+        finalize_range<kBlock>(f2, b, f2.partial2 + 4 * (size_t)b * f2.nf, parts_of(f2, b));
+        return;
+    }
+    b -= nred;
+    if (b < nfin) {
+        finalize_part(f1, b / f1.nf, b % f1.nf);
+        return;
+    }
+    b -= nfin;
+    const int lane = threadIdx.x & 63;
+    const int c0 = (b * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kResFixCells;
+    // lane t < kResFixCells: cell c0 + t's mark, members, arrivals, system and
+    // first slot, all in one round trip
+    const bool mine = lane < kResFixCells && c0 + lane < p.ncell_total;
+    const int cl = mine ? c0 + lane : 0;
+    const int dirty = mine ? p.rc_dirty[cl] : 0;
+    const int n0 = p.rc_count[cl], nar = p.rc_narr[cl], ks = p.cell_sys[cl], rb = p.rc_base[cl];
+    unsigned long long marked = __ballot(dirty != 0);
+    while (marked) {
+        const int t = __builtin_ctzll(marked);
+        marked &= marked - 1;
+        res_fix_cell<MIXED_>(p, c0 + t, __builtin_amdgcn_readlane(ks, t),
+                             __builtin_amdgcn_readlane(rb, t), __builtin_amdgcn_readlane(n0, t),
+                             min(__builtin_amdgcn_readlane(nar, t), kResArr), lane);
+    }
+}
+
+// Packs resident set p.pos etc. into the packed layout of set p.pos_n etc.:
+// cell c's members go to start[c] + rank (start: the scan of rc_count), with
+// their cell-local coordinates and cells, as a sort leaves them.
+template <bool MIXED_>
+__global__ __launch_bounds__(kBlock) void k_res_leave(StepArgs p)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (c >= p.ncell_total) return;
+    const SysDev& S = p.sys[p.cell_sys[c]];
+    if (c - S.cell_off >= S.ncell) return;   // tail cell: padding slots
+    const int n = p.rc_count[c];
+    if (lane >= n) return;
+    const int src = p.rc_base[c] + lane, dst = p.start[c] + lane;
+    p.pos_n.x[dst] = p.pos.x[src]; p.pos_n.y[dst] = p.pos.y[src]; p.pos_n.z[dst] = p.pos.z[src];
+    p.vel_n.x[dst] = p.vel.x[src]; p.vel_n.y[dst] = p.vel.y[src]; p.vel_n.z[dst] = p.vel.z[src];
+    p.disp_n.x[dst] = p.disp.x[src]; p.disp_n.y[dst] = p.disp.y[src];
+    p.disp_n.z[dst] = p.disp.z[src];
+    p.id_n[dst] = p.id[src];
+    p.scell[dst] = c;
+    if (MIXED_) p.lpos_n[dst] = p.lpos[src];
+}
+
+// ---------------------------------------------------------------------------
 // 6. pair forces (full shell, one thread per sorted particle), fused with the
 //    second half kick and the per-block sums of |v|^2, PE, W, |d|^2.  The
 //    pair kernels' KICK: 0 forces only; 1 second half kick (the force is
@@ -931,6 +1169,64 @@ __device__ __forceinline__ void kick_store(const StepArgs& p, const SysDev& S, i
         p.vel.x[s] = vx; p.vel.y[s] = vy; p.vel.z[s] = vz;
     }
 }
+
+// The owner of slot si of home cell c in a resident step's pair kernel, with
+// its new force: the second half kick (vx, vy, vz become the full-step
+// velocity, for the kinetic energy), then the next step's first half kick,
+// drift and wrap with the arithmetic of kick_store, the key pass and the sort
+// (chained steps).  Velocity and displacement are updated in place; the new
+// position goes to the next position set (FP64 neighbours still read the
+// current one) and, in mixed precision, the cell-local coordinates to the
+// next lpos set.  A particle whose new cell differs is handed to the fix-up
+// as an arrival record of its new cell (its slot is vacated: id -1); both
+// cells are marked.  d: the displacement before the drift.
+template <bool MIXED_>
+__device__ __forceinline__ void res_kick_drift(const StepArgs& p, const SysDev& S, int c, int si,
+                                               double fx, double fy, double fz, double& vx,
+                                               double& vy, double& vz, double ddx, double ddy,
+                                               double ddz)
+{
+    const double dt = S.dt;
+    const double hx = ((1. / 2.) * fx) * dt, hy = ((1. / 2.) * fy) * dt,
+                 hz = ((1. / 2.) * fz) * dt;
+    vx = vx + hx;
+    vy = vy + hy;
+    vz = vz + hz;
+    double wx = vx + hx, wy = vy + hy, wz = vz + hz;
+    double x = p.pos.x[si], y = p.pos.y[si], z = p.pos.z[si];
+    double ix, iy, iz;
+    advance<false>(S, x, y, z, wx, wy, wz, 0., 0., 0., ix, iy, iz);
+    const double nx = ddx + ix, ny = ddy + iy, nz = ddz + iz;
+    const int kx = cell_axis(x, S.q[0], S.nc[0]);
+    const int ky = cell_axis(y, S.q[1], S.nc[1]);
+    const int kz = cell_axis(z, S.q[2], S.nc[2]);
+    const int cn = S.cell_off + (kx * S.nc[1] + ky) * S.nc[2] + kz;
+    p.vel.x[si] = wx; p.vel.y[si] = wy; p.vel.z[si] = wz;
+    if (cn == c) {
+        p.disp.x[si] = nx; p.disp.y[si] = ny; p.disp.z[si] = nz;
+        p.pos_n.x[si] = x; p.pos_n.y[si] = y; p.pos_n.z[si] = z;
+        if (MIXED_)
+            p.lpos_n[si] = make_float4((float)(x - kx * S.a[0]), (float)(y - ky * S.a[1]),
+                                       (float)(z - kz * S.a[2]), 0.0f);
+        return;
+    }
+    const int a = atomicAdd(p.rc_narr + cn, 1);
+    if (a < kResArr) {
+        ResMover& mv = p.rc_arr[(size_t)cn * kResArr + a];
+        mv.x = x; mv.y = y; mv.z = z;
+        mv.vx = wx; mv.vy = wy; mv.vz = wz;
+        mv.dx = nx; mv.dy = ny; mv.dz = nz;
+        mv.id = p.id[si];
+        mv.cell = cn;
+    } else {
+        atomicOr(p.rc_flag, 1);
+    }
+    p.id[si] = -1;   // the slot is vacated
+    res_mark(p, c);
+    res_mark(p, cn);
+}
+
+
 
 template <int PREC, int LAW, int KICK>
 __global__ __launch_bounds__(kBlock) void k_pair(StepArgs p)
@@ -1196,10 +1492,11 @@ __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 // waves, wave sub taking the neighbour cells q with q % NWC == sub; their
 // partial forces are added in wave order before the kick.  The grid then runs
 // cpb workgroups per cell-space block.
-template <int LAW, int KICK, int NW, int NWC = 1>
+template <int LAW, int KICK, int NW, int NWC = 1, bool RES = false>
 __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs p)
 {
     static_assert(NW == 1 || NWC == 1, "several waves per cell: one cell per block");
+    static_assert(!RES || (KICK == 2 && NWC == 1), "resident cells: chained steps, one wave per cell");
     // One record per wave, so that every per-lane LDS address is one base
     // register plus an immediate offset.
     struct alignas(16) WaveLds {
@@ -1235,7 +1532,7 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
         int cx, cy, cz;
         cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
-        const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
+        const int h1 = __builtin_amdgcn_readfirstlane(p.cend[c]);
         float4* sw = W.st;
         const float rc2f = S.rc2f;
         const float inv_s2 = S.inv_s2f;
@@ -1252,7 +1549,7 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
             nb_axis(cz, oz, S.nc[2], nbz, iz);
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
-            cntq = (NWC == 1 || lane % NWC == sub) ? p.start[nb + 1] - j0 : 0;
+            cntq = (NWC == 1 || lane % NWC == sub) ? p.cend[nb] - j0 : 0;
             // kept in LDS for the chunks of this cell (not in registers)
             W.tb_img[lane] = make_float4(S.shf[0][3 * (ox + 1) + ix + 1],
                                             S.shf[1][3 * (oy + 1) + iy + 1],
@@ -1269,6 +1566,8 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
         // offset of the home cell (offset 13) in the concatenation
         const int off_home = __builtin_amdgcn_readfirstlane(__shfl(incl - cntq, 13, 64));
         const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        bool res_own = false;   // resident step: this lane owns a particle (force in res_f)
+        float3 res_f = make_float3(0.f, 0.f, 0.f);
 
         for (int i0 = h0; i0 < h1; i0 += 64) {
             const int ni = min(64, h1 - i0);
@@ -1509,16 +1808,37 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
                 const double fx = fscale * (double)ax, fy = fscale * (double)ay,
                              fz = fscale * (double)az;
                 if (p.ncount) p.ncount[si] = nct;
-                if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
-                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-                if (KICK) kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
-                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
-                wacc[0] += (vx * vx + vy * vy) + vz * vz;
-                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
+                if constexpr (RES) {
+                    // resident cells hold at most kResCap < 64 members: this is
+                    // the only chunk, and the drift follows the loop (fewer
+                    // registers live there)
+                    res_own = true;
+                    res_f = make_float3(ax, ay, az);
+                } else {
+                    if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
+                    double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+                    if (KICK) kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
+                    const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                    wacc[0] += (vx * vx + vy * vy) + vz * vz;
+                    wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
+                }
             }
             if (act) {
                 wacc[1] += (1. / 2.) * pe;
                 wacc[2] += (1. / 2.) * w;
+            }
+        }
+        if constexpr (RES) {
+            if (res_own) {
+                constexpr double fscale = (LAW == LAW_LJ) ? 24.0 : 1.0;
+                const int si = h0 + lane;   // owners: lanes 0 .. n - 1 of the one chunk
+                double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+                const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                res_kick_drift<true>(p, S, c, si, fscale * (double)res_f.x,
+                                     fscale * (double)res_f.y, fscale * (double)res_f.z, vx, vy,
+                                     vz, ddx, ddy, ddz);
+                wacc[0] += (vx * vx + vy * vy) + vz * vz;
+                wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
             }
         }
     }
@@ -1575,9 +1895,10 @@ __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 // neighbour cells q with q % NWC == sub; their partial forces are added in
 // wave order before the kick.  The system's flag names the kernel: tiled 1
 // for NWC = 1, 2 for NWC > 1.
-template <int LAW, int KICK, int NWC = 1>
+template <int LAW, int KICK, int NWC = 1, bool RES = false>
 __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells_fp64(StepArgs p)
 {
+    static_assert(!RES || (KICK == 2 && NWC == 1), "resident cells: chained steps, one wave per cell");
     static_assert(kCellsPerBlockFp64 == 1 || NWC == 1, "several waves per cell: one cell per block");
     constexpr int NW = kCellsPerBlockFp64 * NWC;
     __shared__ float4 st[NW][kRowsD];
@@ -1607,7 +1928,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
         int cx, cy, cz;
         cell_xyz_of(p.cell_xyz, c, cx, cy, cz);
         const int h0 = __builtin_amdgcn_readfirstlane(p.start[c]);
-        const int h1 = __builtin_amdgcn_readfirstlane(p.start[c + 1]);
+        const int h1 = __builtin_amdgcn_readfirstlane(p.cend[c]);
         float4* sw = st[wid];
         const double rc2d = S.rc2;
         const double sig2 = S.sigma2;
@@ -1627,7 +1948,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
             nb_axis(cz, oz, S.nc[2], nbz, iz);
             const int nb = S.cell_off + (nbx * S.nc[1] + nby) * S.nc[2] + nbz;
             j0 = p.start[nb];
-            cntq = (NWC == 1 || lane % NWC == sub) ? p.start[nb + 1] - j0 : 0;
+            cntq = (NWC == 1 || lane % NWC == sub) ? p.cend[nb] - j0 : 0;
             sx = ix * S.L[0];
             sy = iy * S.L[1];
             sz = iz * S.L[2];
@@ -1896,8 +2217,11 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
                 if (p.ncount) p.ncount[si] = nct;
                 if (KICK != 2) { p.force.x[si] = fx; p.force.y[si] = fy; p.force.z[si] = fz; }
                 double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-                if (KICK) kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
                 const double ddx = p.disp.x[si], ddy = p.disp.y[si], ddz = p.disp.z[si];
+                if constexpr (RES)
+                    res_kick_drift<false>(p, S, c, si, fx, fy, fz, vx, vy, vz, ddx, ddy, ddz);
+                else if (KICK)
+                    kick_store(p, S, si, fx, fy, fz, vx, vy, vz, KICK == 2);
                 wacc[0] += (vx * vx + vy * vy) + vz * vz;
                 wacc[3] += (ddx * ddx + ddy * ddy) + ddz * ddz;
             }

@@ -85,6 +85,8 @@ def engine():
         "lj_engine_set_active": (c_i, [V, c_i, c_i]),
         "lj_engine_adjoint": (c_i, [V, c_i, c_i, c_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
+        "lj_engine_set_resident_cells": (c_i, [V, c_i]),
+        "lj_engine_resident_stats": (c_i, [V, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
         "lj_engine_set_pair_split": (c_i, [V, c_d]),
         "lj_engine_pair_kernels": (c_i, [V, P_i]),

@@ -222,6 +222,18 @@ class Engine:
     def set_graphs(self, enabled):
         check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
 
+    def set_resident_cells(self, enabled):
+        """Resident cells for step calls of 4 or more steps (default on): the
+        cell lists are built once per call and only the particles that change
+        cell are moved afterwards; results are the same bit for bit."""
+        check(self.lib.lj_engine_set_resident_cells(self.h, 1 if enabled else 0))
+
+    def resident_stats(self):
+        """(calls run on resident cells, calls redone with the full cell build)."""
+        c, f = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.lj_engine_resident_stats(self.h, ctypes.byref(c), ctypes.byref(f)))
+        return c.value, f.value
+
     def set_pair_kernel(self, variant):
         """'tiled' (one wave per home cell, LDS-staged neighbours), 'particle'
         (one thread per particle), 'multiwave' (the tiled kernel with four

@@ -1,0 +1,127 @@
+"""Resident cells: step calls that keep every cell in a fixed block of slots and
+move only the particles that change cell must leave records and state equal,
+bit for bit, to the same calls with the full cell build every step (the
+reference's per-step cell lists, engine/particle_pair_filter.cpp:141-156).
+Calls that exceed a capacity are redone with the full build: same results."""
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "disp", "force")
+
+
+def _eng():
+    from lennardjonesium import engine as eng
+    assert eng.device_count() > 0, "no HIP device"
+    return eng
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _systems(eng, specs):
+    """(SystemSpec, pos, vel) per (n, rho, T, seed)."""
+    out = []
+    for n, rho, T, seed, rc in specs:
+        pos, vel, L = eng.initial_condition(n, rho, T, seed=seed)
+        out.append((eng.SystemSpec(n, L, cutoff=rc), pos, vel))
+    return out
+
+
+def _run(eng, systems, precision, calls, resident, graphs=True):
+    e = eng.Engine([s for s, _, _ in systems], precision=precision)
+    e.set_graphs(graphs)
+    e.set_resident_cells(resident)
+    for k, (_, p, v) in enumerate(systems):
+        e.set_state(k, p, v)
+    recs = [e.step(n) for n in calls]
+    states = [e.get_state(k) for k in range(len(systems))]
+    stats = e.resident_stats()
+    e.close()
+    return np.concatenate(recs), states, stats
+
+
+def _same(a, b):
+    ra, sa, _ = a
+    rb, sb, _ = b
+    assert ra.shape == rb.shape
+    assert np.array_equal(ra, rb), f"records differ: max {np.abs(ra - rb).max():.3e}"
+    for x, y in zip(sa, sb):
+        for f in FIELDS:
+            assert np.array_equal(x[f], y[f]), f"{f} differs: max {np.abs(x[f] - y[f]).max():.3e}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_equals_full_build_batch(precision):
+    """Three systems in one engine (a dense liquid, a hot fluid with many
+    cell changes per step, a small box): one call of 300 steps."""
+    eng = _eng()
+    systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5), (13500, 0.6, 2.0, 7, 2.5),
+                             (4000, 0.8, 1.0, 11, 2.5)])
+    _say("resident vs full build, batch of 3,", precision)
+    a = _run(eng, systems, precision, [300], True)
+    b = _run(eng, systems, precision, [300], False)
+    assert a[2] == (1, 0), f"resident calls / fallbacks: {a[2]}"
+    assert b[2] == (0, 0)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_split_calls_and_eager(precision):
+    """Calls of 5, 37, 4, 3 (below the resident minimum: full build), 2 and
+    100 steps, replayed from graphs and launched eagerly, equal one call of
+    151 steps with the full build."""
+    eng = _eng()
+    systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5)])
+    _say("resident split calls,", precision)
+    ref = _run(eng, systems, precision, [151], False)
+    for graphs in (True, False):
+        a = _run(eng, systems, precision, [5, 37, 4, 3, 2, 100], True, graphs=graphs)
+        assert a[2] == (4, 0), f"resident calls / fallbacks: {a[2]}"
+        _same(a, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_liquid_256k(precision):
+    """The N = 256,000 liquid (rho 0.8442, T 0.72): 200 steps."""
+    eng = _eng()
+    systems = _systems(eng, [(256000, 0.8442, 0.72, 5489, 2.5)])
+    _say("resident liquid 256k,", precision)
+    a = _run(eng, systems, precision, [200], True)
+    b = _run(eng, systems, precision, [200], False)
+    assert a[2] == (1, 0)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_coexistence_2048k(precision):
+    """The N = 2,048,000 coexistence workload (rho 0.30, T 0.80, rc 3.0, dt
+    0.004; 250,047 cells, so the packing scan runs in two launches): 30 steps."""
+    eng = _eng()
+    n = 2048000
+    pos, vel, L = eng.initial_condition(n, 0.30, 0.80, seed=5489)
+    systems = [(eng.SystemSpec(n, L, cutoff=3.0, time_delta=0.004), pos, vel)]
+    _say("resident coexistence 2048k,", precision)
+    a = _run(eng, systems, precision, [30], True)
+    b = _run(eng, systems, precision, [30], False)
+    assert a[2] == (1, 0)
+    _same(a, b)
+
+
+def test_resident_fallback_when_cells_overflow():
+    """A cutoff of 5 sigma at rho 0.8442 puts about 105 particles in a cell,
+    beyond the 56 resident slots: the call is redone with the full build and
+    its results equal a run with resident cells off."""
+    eng = _eng()
+    systems = _systems(eng, [(4000, 0.8442, 0.72, 3, 5.0)])
+    _say("resident fallback")
+    a = _run(eng, systems, "mixed", [20, 20], True)
+    b = _run(eng, systems, "mixed", [20, 20], False)
+    assert a[2] == (2, 2), f"resident calls / fallbacks: {a[2]}"
+    _same(a, b)
