@@ -160,21 +160,26 @@ def energy_gate(eng, specs, states, precision, pair_kernel="tiled"):
     return out
 
 
-PER_CALL_KERNELS = ("finalize",)   # run once per call, not once per step
-
-
 def step_kernel_ms(kt):
-    """Sum of the per-step kernels' mean durations (ms)."""
-    return sum(v for k, v in kt.items() if k not in PER_CALL_KERNELS)
+    """Kernel time per step (ms): the kernels launched once per step (a
+    resident call's cell build and packing, and the final record sums, run
+    once per call and are left out)."""
+    return kt["per_step_ms"]
 
 
 def kernel_times(e, steps=40):
-    """Mean duration of each step kernel (HIP events on the engine's stream)."""
+    """Mean duration of each kernel (HIP events on the engine's stream) over
+    one call of `steps` steps, its launches, and the per-step kernels' time
+    per step."""
     e.set_profiling(True)
     e.step(steps)
     kt = e.kernel_times()
+    nl = e.kernel_launches()
     e.set_profiling(False)
-    return kt
+    per_step = {k for k, n in nl.items() if n >= steps - 2}
+    return {"mean_ms": kt, "launches": nl, "steps": steps,
+            "per_step_ms": sum(kt[k] * nl[k] for k in per_step) / steps,
+            "per_call_ms": {k: kt[k] * nl[k] for k in kt if k not in per_step}}
 
 
 def read_profile_value(name, key):
@@ -256,6 +261,7 @@ def liquid(args, eng):
     log(f"mixed: {tm:.4f} s, {rate_m:.4e} atom-steps/s")
     ktm = kernel_times(em)
     log("mixed kernel ms:", json.dumps(ktm))
+    resident = em.resident_stats()[0] > 0
     st = em.get_state(0)
     res64, frms, kt64 = None, None, None
     if not args.no_fp64:
@@ -287,13 +293,14 @@ def liquid(args, eng):
                 f"10^4-step rate {g['rate_10000_steps']:.4e}")
     c_half, p_half = pair_counts(eng, st["pos"], L)
     flops = pair_flops(c_half, p_half)
-    achieved = flops / (ktm["pair"] * 1e-3) / 1e12
+    pair_ms = ktm["mean_ms"]["pair"]
+    achieved = flops / (pair_ms * 1e-3) / 1e12
     traffic = read_traffic("pair_mixed_256k_traffic.txt")
     # VALU issue: wave-instructions per launch (committed SQ counters) over the
     # CUs' issue capacity in the measured duration (one wave-instruction per CU
     # per clock at 2.4 GHz, 256 CUs)
     valu = read_profile_value("step_kernels_256k_sq_counters.md", "valu_insts_per_launch")
-    issue = valu / (ktm["pair"] * 1e-3 * 256 * 2.4e9) if valu else None
+    issue = valu / (pair_ms * 1e-3 * 256 * 2.4e9) if valu else None
     ms_m = 1e3 * tm / args.steps
     roof = {"bound": "valu", "executes_on": "VALU (FP32 vector; no MFMA: pair forces are "
                                            "not a dense contraction)",
@@ -301,17 +308,20 @@ def liquid(args, eng):
             "frac": achieved / PEAK["fp32"], "traffic": traffic,
             "peak_unpacked_fp32": PEAK["fp32_unpacked"],
             "frac_of_unpacked": achieved / PEAK["fp32_unpacked"],
-            "kernel": ("k_pair_cells_mixed<LJ,KICK>" if args.pair_kernel == "tiled"
-                       else "k_pair<MIXED,LJ,KICK>"),
-            "kernel_ms": ktm["pair"], "flops_per_launch": flops,
+            "kernel": (("k_pair_cells_mixed<LJ,2,4,1,RES> (pair forces, kicks, drift of "
+                        "resident cells)" if resident else "k_pair_cells_mixed<LJ,KICK>")
+                       if args.pair_kernel == "tiled" else "k_pair<MIXED,LJ,KICK>"),
+            "kernel_ms": pair_ms, "flops_per_launch": flops,
             "c_half": c_half, "p_half": p_half,
             "valu_insts_per_launch": valu, "valu_issue_frac": issue,
             "step": step_roofline(flops, 224.0 * n, ms_m, PEAK["fp32"])}
     if res64:
-        a64 = flops / (kt64["pair"] * 1e-3) / 1e12
-        res64["roofline"] = {"bound": "valu", "kernel": "k_pair_cells_fp64<LJ,KICK>",
+        a64 = flops / (kt64["mean_ms"]["pair"] * 1e-3) / 1e12
+        res64["roofline"] = {"bound": "valu",
+                             "kernel": ("k_pair_cells_fp64<LJ,2,1,RES>" if resident
+                                        else "k_pair_cells_fp64<LJ,KICK>"),
                              "achieved": a64, "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                             "frac": a64 / PEAK["fp64"], "kernel_ms": kt64["pair"],
+                             "frac": a64 / PEAK["fp64"], "kernel_ms": kt64["mean_ms"]["pair"],
                              "step": step_roofline(flops, 200.0 * n, res64["ms_per_step"],
                                                    PEAK["fp64"])}
         res64["launch_gaps_us_per_step"] = 1e3 * (gates["fp64"]["ms_per_step_10000_steps"] -
@@ -332,7 +342,8 @@ def liquid(args, eng):
                  f"advanced {args.prepare} untimed NVE steps into the liquid (input preparation, "
                  f"before the warm-up; the lattice melts within them) in each precision"),
         "config": {"workload": "NVE LJ liquid N=256,000 (FCC 40^3), rho=0.8442, T=0.72, "
-                               "rc=2.5, dt=0.005, cell list rebuilt every step, mixed precision",
+                               "rc=2.5, dt=0.005, cell lists updated every step, mixed precision",
+                   "resident_cells": resident,
                    "prepared_steps": args.prepare,
                    "particles": n, "steps": args.steps, "precision": "mixed",
                    "pair_kernel": args.pair_kernel, "parallelism": "single GPU"},
@@ -570,7 +581,7 @@ def coexistence(args, eng):
     return {"metric": METRIC, "value": m["value"], "unit": "atom-steps/s", "n_gpus": 1,
             "steps": steps, "dtype": "f32 pair forces + f64 state/sums", "data": "synthetic",
             "config": {"workload": "coexistence N=2,048,000 (FCC 80^3) rho 0.30 T 0.80 rc 3.0 "
-                                   "dt 0.004, cell list rebuilt every step", "pair_kernel":
+                                   "dt 0.004, cell lists updated every step", "pair_kernel":
                        m["pair_kernel"]},
             "mixed": m, "fp64": out.get("fp64"),
             "mixed_over_fp64": (m["value"] / out["fp64"]["value"]) if "fp64" in out else None,

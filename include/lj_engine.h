@@ -226,19 +226,28 @@ int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double density, doubl
 /* 0 disables HIP graph capture of the step loop (launches each kernel). */
 int lj_engine_set_graphs(lj_engine* e, int enabled);
 
-/* Resident cells (default on): a step call of at least 4 steps on an engine
- * whose systems all use the one-wave tiled pair kernel (LJ_PAIR_CELL_TILED)
- * builds its cell lists once, keeps every cell in a fixed block of slots and,
- * in each later step, moves only the particles that changed cell (in place of
- * the key pass, scan and sort); the call's last step packs the state back.
- * Records and state are the same bit for bit as with it off.  A call in which
- * a cell would exceed its slots (56 members) or receive more than 8 particles
- * in one step is redone from its starting state with the full cell build
- * (same results, more time).  The first such call allocates the resident
- * buffers (about 190 bytes per cell slot, 56 slots per cell).  No counterpart
- * in the reference, which rebuilds its cell lists every step
- * (engine/particle_pair_filter.cpp:141-156). */
-int lj_engine_set_resident_cells(lj_engine* e, int enabled);
+/* Resident cells: a step call of at least 4 steps on an engine whose systems
+ * all use the tiled pair kernel (all LJ_PAIR_CELL_TILED or all
+ * LJ_PAIR_CELL_MULTIWAVE) builds its cell lists once, keeps every cell in a
+ * fixed block of slots and, in each later step, moves only the particles that
+ * changed cell (in place of the key pass, scan and sort); the call's last step
+ * packs the state back.  Records and state are the same bit for bit with any
+ * mode.  A call in which a cell would exceed its slots (56 members) or receive
+ * more than 8 particles in one step is redone from its starting state with
+ * the full cell build (same results, more time).  The first such call
+ * allocates the resident buffers (about 190 bytes per cell slot, 56 slots per
+ * cell).  Modes: LJ_RESIDENT_OFF; LJ_RESIDENT_AUTO (default): when the engine
+ * holds at least 10 particles per cell (liquid densities; dilute systems gain
+ * nothing); LJ_RESIDENT_ON: whenever the kernels allow it.  lj_engine_autotune
+ * also chooses between on and off.  No counterpart in the reference, which
+ * rebuilds its cell lists every step (engine/particle_pair_filter.cpp:141-156). */
+#define LJ_RESIDENT_OFF 0
+#define LJ_RESIDENT_AUTO 1
+#define LJ_RESIDENT_ON 2
+int lj_engine_set_resident_cells(lj_engine* e, int mode);
+
+/* The resident-cell mode (LJ_RESIDENT_OFF for a null engine). */
+int lj_engine_resident_cells(const lj_engine* e);
 
 /* Calls run on resident cells since creation, and how many of them were
  * redone with the full cell build. */
@@ -279,7 +288,9 @@ int lj_engine_launch_shape(lj_engine* e, int* cells_per_block, int* sort_wave_pa
 
 /* Project autotuner, for the reference's one-Simulation-per-thread pool
  * (api/simulation_pool.cpp:93-101) that this engine's batches replace.  It
- * times `steps` steps of each pair-kernel assignment -- all tiled, all
+ * times calls of `steps` and 2 `steps` steps of each pair-kernel assignment
+ * (their difference: the time per step of long calls, without what a call
+ * costs once) -- all tiled, all
  * per-particle, and per-system splits at cell occupancies 1.5, 3, 6 and 12
  * that give distinct assignments -- then, with the fastest assignment, each
  * launch shape (cells per block 2 / 4 / 8 for mixed tiled systems, sort waves
@@ -295,6 +306,11 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
  * stream).  names/ms receive up to cap entries; returns the entry count. */
 int lj_engine_set_profiling(lj_engine* e, int enabled);
 int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms, int* count);
+
+/* Launch count of each kernel of lj_engine_kernel_times, in the same order
+ * (per-step kernels launch once per step, the cell build of a resident call
+ * once per call). */
+int lj_engine_kernel_launches(lj_engine* e, int cap, long long* launches, int* count);
 
 #ifdef __cplusplus
 }

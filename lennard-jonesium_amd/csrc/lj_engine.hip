@@ -179,7 +179,7 @@ struct lj_engine {
 
     // Resident cells (chained steps without a cell build; lj_kernels.hpp
     // section 4b): buffers allocated on first use, graphs of resident steps.
-    bool res_enabled = true;
+    int res_mode = LJ_RESIDENT_AUTO;
     bool res_ready = false;
     int res_nslot = 0;                 // kResCap slots per real cell
     int res_nfix = 0;                  // fix-up blocks (the listed cells, one wave each)
@@ -1244,7 +1244,8 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Resident cells (lj_kernels.hpp section 4b).  A call of n >= kResMinSteps
-// steps on an engine whose systems all use the one-wave tiled kernel runs
+// steps on an engine whose systems all use the tiled kernel (all with one
+// wave per cell, or all with several) runs
 //   step 1:      key pass -> k_res_enter (into the resident slots) -> pair -> drift -> fix-up
 //   steps 2..n-1: pair -> drift -> fix-up (replayed from graphs of resident steps)
 //   step n:      scan of the resident counts -> k_res_leave (packed set) -> pair
@@ -1255,16 +1256,27 @@ namespace {
 // checkpointed at its start).
 // ---------------------------------------------------------------------------
 constexpr int kResMinSteps = 4;
+// LJ_RESIDENT_AUTO: particles per cell at which resident cells are used.
+// Measured on the liquid (14.6 per cell: step 74.4 -> 64.7 us) and on the
+// coexistence workload (8.2 per cell, half of the cells nearly empty: 521 ->
+// 599 us, the fix-up and the per-cell drift outweighing the saved sort).
+constexpr double kResMinOccupancy = 10.0;
 
 bool res_eligible(const lj_engine* e, int nsteps)
 {
-    if (!e->res_enabled || !e->has_cells || nsteps < kResMinSteps) return false;
-    if (e->n_tiled != e->nsys) return false;   // one-wave tiled kernel for every system
-    long long slots = 0;
+    if (e->res_mode == LJ_RESIDENT_OFF || !e->has_cells || nsteps < kResMinSteps) return false;
+    // the tiled kernel for every system, with one or with several waves per cell
+    if (e->n_tiled != e->nsys && e->n_multi != e->nsys) return false;
+    long long slots = 0, n = 0, cells = 0;
     for (const SysDev& s : e->sys) {
         if (!s.active || s.law == ljk::LAW_NONE) return false;
         slots += (long long)s.ncell * ljk::kResCap;
+        n += s.n;
+        cells += s.ncell;
     }
+    // the drift's and the fix-up's work per cell pays off at liquid occupancies
+    if (e->res_mode == LJ_RESIDENT_AUTO && (double)n < kResMinOccupancy * (double)cells)
+        return false;
     // resident slots: within the slot limit and at most 16 per particle
     return slots <= kMaxSlots && slots <= 16LL * e->nslot + (1 << 20);
 }
@@ -1331,6 +1343,14 @@ template <int LAW>
 void launch_res_pair(lj_engine* e, const ljk::StepArgs& p)
 {
     const dim3 grid(e->ncblk);
+    if (e->n_multi == e->nsys) {   // several waves per home cell (small systems)
+        const dim3 gm(e->ncblk * p.cpb), bm(64 * ljk::kWavesPerCellSmall);
+        if (e->precision == LJ_PRECISION_MIXED)
+            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 1, ljk::kWavesPerCellSmall, true>, gm, bm, p);
+        else
+            launch(e, ljk::k_pair_cells_fp64<LAW, 2, ljk::kWavesPerCellSmall, true>, gm, bm, p);
+        return;
+    }
     if (e->precision == LJ_PRECISION_MIXED) {
         if (e->cells_per_block == 2)
             launch(e, ljk::k_pair_cells_mixed<LAW, 2, 2, 1, true>, grid, dim3(128), p);
@@ -1814,12 +1834,19 @@ int lj_engine_set_graphs(lj_engine* e, int enabled)
     return LJ_OK;
 }
 
-int lj_engine_set_resident_cells(lj_engine* e, int enabled)
+int lj_engine_set_resident_cells(lj_engine* e, int mode)
 {
     int rc = check_engine(e);
     if (rc) return rc;
-    e->res_enabled = enabled != 0;
+    if (mode != LJ_RESIDENT_OFF && mode != LJ_RESIDENT_AUTO && mode != LJ_RESIDENT_ON)
+        return fail(LJ_E_INVALID, "resident cells: mode must be LJ_RESIDENT_OFF, _AUTO or _ON");
+    e->res_mode = mode;
     return LJ_OK;
+}
+
+int lj_engine_resident_cells(const lj_engine* e)
+{
+    return e ? e->res_mode : LJ_RESIDENT_OFF;
 }
 
 int lj_engine_resident_stats(const lj_engine* e, long long* calls, long long* fallbacks)
@@ -1894,7 +1921,7 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         if (chosen) *chosen = e->pair_variant;
         return LJ_OK;
     }
-    if (steps < 2) steps = 2;
+    if (steps < kResMinSteps) steps = kResMinSteps;   // both trial calls alike (resident or not)
     // save the state (current buffer set, forces, clocks) so trials leave no trace
     const size_t ns = e->nslot;
     DevBuf sp, sv, sd, sf, st;
@@ -1944,11 +1971,13 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     // every exit after the save goes through here: the state and the kernel
     // policy the engine had on entry are put back unless a winner was chosen
     const int cpb_entry = e->cells_per_block, sw_entry = e->sort_wave;
+    const int res_entry = e->res_mode;
     auto finish = [&](int code, bool restored) {
         if (!restored) (void)restore();
         (void)hipFree(sid);
         if (code) {
             const std::string msg = g_err;
+            e->res_mode = res_entry;
             e->split_occupancy = occ0;
             e->pair_variant = variant0;
             (void)lj_engine_set_launch_shape(e, cpb_entry, sw_entry);
@@ -1978,24 +2007,47 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
             cands.push_back({LJ_PAIR_BY_OCCUPANCY, thr});
         }
     }
+    // One candidate: calls of `steps` and of 2 `steps` steps, timed with
+    // events; their difference over `steps` is the time per step of long
+    // calls (what a call costs once, such as a resident call's cell build and
+    // packing, cancels out).
+    auto trial = [&](double* per_step) -> int {
+        int trc;
+        if ((trc = lj_engine_reserve(e, 2 * steps))) return trc;   // graphs built untimed
+        if ((trc = lj_engine_step(e, 2, nullptr))) return trc;     // warm-up
+        float ms[2] = {0, 0};
+        for (int r = 0; r < 2; ++r) {
+            hipError_t h = hipEventRecord(e->ev0, e->stream);
+            if (h != hipSuccess) return fail(LJ_E_HIP, "autotune: event record failed");
+            if ((trc = lj_engine_step(e, (r + 1) * steps, nullptr))) return trc;
+            h = hipEventRecord(e->ev1, e->stream);
+            if (h == hipSuccess) h = hipEventSynchronize(e->ev1);
+            if (h == hipSuccess) h = hipEventElapsedTime(&ms[r], e->ev0, e->ev1);
+            if (h != hipSuccess) return fail(LJ_E_HIP, "autotune: event timing failed");
+        }
+        *per_step = std::max(0.0, (double)(ms[1] - ms[0])) / steps;
+        return LJ_OK;
+    };
     double best = 1e300;
     Candidate best_c = {variant0, occ0};
+    int best_res = e->res_mode;
     for (const Candidate& c : cands) {
         e->split_occupancy = c.variant == LJ_PAIR_BY_OCCUPANCY ? c.occupancy : occ0;
         if ((rc = lj_engine_set_pair_kernel(e, c.variant))) return finish(rc, false);
-        if ((rc = lj_engine_reserve(e, steps))) return finish(rc, false);   // graphs built untimed
-        if ((rc = lj_engine_step(e, 2, nullptr))) return finish(rc, false);   // warm-up
-        hipError_t h = hipEventRecord(e->ev0, e->stream);
-        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event record failed"), false);
-        if ((rc = lj_engine_step(e, steps, nullptr))) return finish(rc, false);
-        float ms = 0;
-        h = hipEventRecord(e->ev1, e->stream);
-        if (h == hipSuccess) h = hipEventSynchronize(e->ev1);
-        if (h == hipSuccess) h = hipEventElapsedTime(&ms, e->ev0, e->ev1);
-        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event timing failed"), false);
-        if (ms < best) { best = ms; best_c = c; }
-        if (restore() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: restore failed"), true);
+        // tiled assignments with and without resident cells
+        for (int res : {LJ_RESIDENT_ON, LJ_RESIDENT_OFF}) {
+            e->res_mode = res;
+            if (res == LJ_RESIDENT_ON && !res_eligible(e, 2 * steps)) continue;
+            double ms = 0;
+            if ((rc = trial(&ms))) { e->res_mode = res_entry; return finish(rc, false); }
+            if (ms < best) { best = ms; best_c = c; best_res = res; }
+            if (restore() != hipSuccess) {
+                e->res_mode = res_entry;
+                return finish(fail(LJ_E_HIP, "autotune: restore failed"), true);
+            }
+        }
     }
+    e->res_mode = best_res;
     e->split_occupancy = best_c.variant == LJ_PAIR_BY_OCCUPANCY ? best_c.occupancy : occ0;
     if ((rc = lj_engine_set_pair_kernel(e, best_c.variant))) return finish(rc, true);
     // then the launch shape with that assignment: waves per workgroup of the
@@ -2012,23 +2064,15 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     for (const auto& sh : shapes) {
         if (sh.first == cpb0 && sh.second == sw0) continue;   // timed above
         if ((rc = lj_engine_set_launch_shape(e, sh.first, sh.second))) return finish(rc, true);
-        if ((rc = lj_engine_reserve(e, steps))) return finish(rc, false);
-        if ((rc = lj_engine_step(e, 2, nullptr))) return finish(rc, false);
-        hipError_t h = hipEventRecord(e->ev0, e->stream);
-        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event record failed"), false);
-        if ((rc = lj_engine_step(e, steps, nullptr))) return finish(rc, false);
-        float ms = 0;
-        h = hipEventRecord(e->ev1, e->stream);
-        if (h == hipSuccess) h = hipEventSynchronize(e->ev1);
-        if (h == hipSuccess) h = hipEventElapsedTime(&ms, e->ev0, e->ev1);
-        if (h != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: event timing failed"), false);
+        double ms = 0;
+        if ((rc = trial(&ms))) return finish(rc, false);
         if (ms < best) { best = ms; best_cpb = sh.first; best_sw = sh.second; }
         if (restore() != hipSuccess) return finish(fail(LJ_E_HIP, "autotune: restore failed"), true);
     }
     if ((rc = lj_engine_set_launch_shape(e, best_cpb, best_sw))) return finish(rc, true);
     (void)finish(LJ_OK, true);
     if (chosen) *chosen = best_c.variant;
-    if (ms_per_step) *ms_per_step = best / steps;
+    if (ms_per_step) *ms_per_step = best;
     return LJ_OK;
 }
 
@@ -2054,6 +2098,21 @@ int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms
             if (names) names[k] = t.name;
             if (ms) ms[k] = t.total_ms / (double)t.calls;
         }
+        ++k;
+    }
+    if (count) *count = k;
+    return LJ_OK;
+}
+
+int lj_engine_kernel_launches(lj_engine* e, int cap, long long* launches, int* count)
+{
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if ((rc = flush_timers(e))) return rc;
+    int k = 0;
+    for (const KernelTimer& t : e->timers) {
+        if (!t.calls) continue;
+        if (k < cap && launches) launches[k] = t.calls;
         ++k;
     }
     if (count) *count = k;

@@ -1496,7 +1496,7 @@ template <int LAW, int KICK, int NW, int NWC = 1, bool RES = false>
 __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs p)
 {
     static_assert(NW == 1 || NWC == 1, "several waves per cell: one cell per block");
-    static_assert(!RES || (KICK == 2 && NWC == 1), "resident cells: chained steps, one wave per cell");
+    static_assert(!RES || KICK == 2, "resident cells: chained steps");
     // One record per wave, so that every per-lane LDS address is one base
     // register plus an immediate offset.
     struct alignas(16) WaveLds {
@@ -1898,7 +1898,7 @@ __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 template <int LAW, int KICK, int NWC = 1, bool RES = false>
 __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells_fp64(StepArgs p)
 {
-    static_assert(!RES || (KICK == 2 && NWC == 1), "resident cells: chained steps, one wave per cell");
+    static_assert(!RES || KICK == 2, "resident cells: chained steps");
     static_assert(kCellsPerBlockFp64 == 1 || NWC == 1, "several waves per cell: one cell per block");
     constexpr int NW = kCellsPerBlockFp64 * NWC;
     __shared__ float4 st[NW][kRowsD];

@@ -86,6 +86,7 @@ def engine():
         "lj_engine_adjoint": (c_i, [V, c_i, c_i, c_d, P_d, P_d, P_d, P_d, P_d]),
         "lj_engine_set_graphs": (c_i, [V, c_i]),
         "lj_engine_set_resident_cells": (c_i, [V, c_i]),
+        "lj_engine_resident_cells": (c_i, [V]),
         "lj_engine_resident_stats": (c_i, [V, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
         "lj_engine_set_pair_kernel": (c_i, [V, c_i]),
         "lj_engine_set_pair_split": (c_i, [V, c_d]),
@@ -95,6 +96,7 @@ def engine():
         "lj_engine_launch_shape": (c_i, [V, P_i, P_i]),
         "lj_engine_set_profiling": (c_i, [V, c_i]),
         "lj_engine_kernel_times": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_char_p), P_d, P_i]),
+        "lj_engine_kernel_launches": (c_i, [V, c_i, ctypes.POINTER(ctypes.c_longlong), P_i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

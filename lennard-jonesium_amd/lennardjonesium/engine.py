@@ -222,11 +222,16 @@ class Engine:
     def set_graphs(self, enabled):
         check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
 
-    def set_resident_cells(self, enabled):
-        """Resident cells for step calls of 4 or more steps (default on): the
-        cell lists are built once per call and only the particles that change
-        cell are moved afterwards; results are the same bit for bit."""
-        check(self.lib.lj_engine_set_resident_cells(self.h, 1 if enabled else 0))
+    def set_resident_cells(self, mode):
+        """Resident cells for step calls of 4 or more steps: the cell lists are
+        built once per call and only the particles that change cell are moved
+        afterwards; results are the same bit for bit.  mode: False / 'off',
+        'auto' (default: at 10 or more particles per cell), True / 'on'."""
+        modes = {False: 0, "off": 0, "auto": 1, True: 2, "on": 2}
+        check(self.lib.lj_engine_set_resident_cells(self.h, modes[mode]))
+
+    def resident_cells(self):
+        return {0: "off", 1: "auto", 2: "on"}[self.lib.lj_engine_resident_cells(self.h)]
 
     def resident_stats(self):
         """(calls run on resident cells, calls redone with the full cell build)."""
@@ -281,6 +286,16 @@ class Engine:
         cnt = ctypes.c_int()
         check(self.lib.lj_engine_kernel_times(self.h, 32, names, _dptr(ms), ctypes.byref(cnt)))
         return {names[k].decode(): float(ms[k]) for k in range(min(cnt.value, 32))}
+
+    def kernel_launches(self):
+        """Launches of each kernel of kernel_times() (same keys)."""
+        names = (ctypes.c_char_p * 32)()
+        ms = np.zeros(32)
+        cnt = ctypes.c_int()
+        check(self.lib.lj_engine_kernel_times(self.h, 32, names, _dptr(ms), ctypes.byref(cnt)))
+        n = (ctypes.c_longlong * 32)()
+        check(self.lib.lj_engine_kernel_launches(self.h, 32, n, ctypes.byref(cnt)))
+        return {names[k].decode(): int(n[k]) for k in range(min(cnt.value, 32))}
 
 
 def device_count():

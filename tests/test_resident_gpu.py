@@ -34,8 +34,8 @@ def _systems(eng, specs):
     return out
 
 
-def _run(eng, systems, precision, calls, resident, graphs=True):
-    e = eng.Engine([s for s, _, _ in systems], precision=precision)
+def _run(eng, systems, precision, calls, resident, graphs=True, kernel="tiled"):
+    e = eng.Engine([s for s, _, _ in systems], precision=precision, pair_kernel=kernel)
     e.set_graphs(graphs)
     e.set_resident_cells(resident)
     for k, (_, p, v) in enumerate(systems):
@@ -69,6 +69,19 @@ def test_resident_equals_full_build_batch(precision):
     b = _run(eng, systems, precision, [300], False)
     assert a[2] == (1, 0), f"resident calls / fallbacks: {a[2]}"
     assert b[2] == (0, 0)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_small_systems_multiwave(precision):
+    """Small systems on the tiled kernel with four waves per home cell (N = 864
+    and 4,000 in one engine): 200 steps."""
+    eng = _eng()
+    systems = _systems(eng, [(864, 0.8, 1.0, 5489, 2.5), (4000, 0.8, 1.0, 17, 2.5)])
+    _say("resident multiwave,", precision)
+    a = _run(eng, systems, precision, [200], True, kernel="multiwave")
+    b = _run(eng, systems, precision, [200], False, kernel="multiwave")
+    assert a[2] == (1, 0), f"resident calls / fallbacks: {a[2]}"
     _same(a, b)
 
 
