@@ -232,10 +232,10 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
  * fixed block of slots and, in each later step, moves only the particles that
  * changed cell (in place of the key pass, scan and sort); the call's last step
  * packs the state back.  Records and state are the same bit for bit with any
- * mode.  A call in which a cell would exceed its slots (56 members) or receive
+ * mode.  A call in which a cell would exceed its slots (48 members) or receive
  * more than 8 particles in one step is redone from its starting state with
  * the full cell build (same results, more time).  The first such call
- * allocates the resident buffers (about 190 bytes per cell slot, 56 slots per
+ * allocates the resident buffers (about 150 bytes per cell slot, 48 slots per
  * cell).  Modes: LJ_RESIDENT_OFF; LJ_RESIDENT_AUTO (default): when the engine
  * holds at least 10 particles per cell (liquid densities; dilute systems gain
  * nothing); LJ_RESIDENT_ON: whenever the kernels allow it.  lj_engine_autotune

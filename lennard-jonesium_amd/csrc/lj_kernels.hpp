@@ -255,9 +255,9 @@ struct StepArgs {
 
 // Resident cells: slots per cell (a cell with more members makes the call
 // fall back to the packed layout) and arrivals per cell and step.
-constexpr int kResCap = 56;
+constexpr int kResCap = 48;
 constexpr int kResArr = 8;
-constexpr int kResFixCells = 2;   // cells per fix-up wave
+constexpr int kResFixCells = 1;   // cells per fix-up wave (1: 5.6 us per step, 2: 6.5, 4: 8.7 on the liquid)
 static_assert(kResCap + kResArr <= 64, "a cell's members and arrivals share one wave");
 
 // A particle that left its cell during a resident step: its integrated state
