@@ -118,6 +118,7 @@ struct lj_engine {
     int cells_per_block = ljk::kCellsPerBlockMixed;   // mixed tiled kernel: waves per workgroup
     int sort_wave = 44;             // sort: target particles per wave
     int pair_variant = LJ_PAIR_CELL_TILED;
+    bool unit_sigma = false;        // every system has sigma = 1 (the tiled kernels skip a multiply)
     bool last_pair_force = false;   // the last pair launch computed forces (not sums only)
     int n_tiled = 0;                // systems whose pair forces come from the tiled kernel
     int n_multi = 0;                // ... from the tiled kernel with several waves per cell
@@ -468,19 +469,35 @@ void launch_pair(lj_engine* e, const ljk::StepArgs& p)
     launch(e, ljk::k_pair<PREC, LAW, KICK>, dim3(grid), dim3(kBlock), p);
 }
 
+// The one-wave tiled kernel of a precision over `grid` blocks, for the launch
+// shape and, when every system has sigma = 1, the law without the sigma^2
+// multiply (bit-identical: the product with 1 is exact).
+template <int PREC, int LAW, int KICK, bool RES>
+void launch_tiled_shape(lj_engine* e, const ljk::StepArgs& p, dim3 grid)
+{
+    auto go = [&](auto unit_t) {
+        constexpr bool U = decltype(unit_t)::value && LAW == ljk::LAW_LJ;
+        if constexpr (PREC == ljk::MIXED) {
+            if (e->cells_per_block == 2)
+                launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 2, 1, RES, U>, grid, dim3(128), p);
+            else if (e->cells_per_block == 8)
+                launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 8, 1, RES, U>, grid, dim3(512), p);
+            else
+                launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 4, 1, RES, U>, grid, dim3(256), p);
+        } else {
+            launch(e, ljk::k_pair_cells_fp64<LAW, KICK, 1, RES, U>, grid,
+                   dim3(64 * ljk::kCellsPerBlockFp64), p);
+        }
+    };
+    if (e->unit_sigma) go(std::true_type{});
+    else go(std::false_type{});
+}
+
 template <int PREC, int LAW, int KICK>
 void launch_pair_cells(lj_engine* e, const ljk::StepArgs& p)
 {
     const int grid = p.cblk_list ? e->n_cblk_list : e->ncblk;
-    if constexpr (PREC == ljk::MIXED) {
-        if (e->cells_per_block == 2)
-            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 2>, dim3(grid), dim3(128), p);
-        else if (e->cells_per_block == 8)
-            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 8>, dim3(grid), dim3(512), p);
-        else
-            launch(e, ljk::k_pair_cells_mixed<LAW, KICK, 4>, dim3(grid), dim3(256), p);
-    } else
-        launch(e, ljk::k_pair_cells_fp64<LAW, KICK>, dim3(grid), dim3(64 * ljk::kCellsPerBlockFp64), p);
+    launch_tiled_shape<PREC, LAW, KICK, false>(e, p, dim3(grid));
 }
 
 // Systems flagged 2 (small systems): the tiled kernel of their precision with
@@ -972,6 +989,8 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         e->sys.push_back(s);
     }
     e->nslot = atom_off;
+    e->unit_sigma = true;
+    for (const SysDev& s : e->sys) e->unit_sigma = e->unit_sigma && s.sigma2 == 1.0;
     for (const SysDev& s : e->sys) {
         e->n_tiled += s.tiled == 1;
         e->n_multi += s.tiled == 2;
@@ -1351,16 +1370,8 @@ void launch_res_pair(lj_engine* e, const ljk::StepArgs& p)
             launch(e, ljk::k_pair_cells_fp64<LAW, 2, ljk::kWavesPerCellSmall, true>, gm, bm, p);
         return;
     }
-    if (e->precision == LJ_PRECISION_MIXED) {
-        if (e->cells_per_block == 2)
-            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 2, 1, true>, grid, dim3(128), p);
-        else if (e->cells_per_block == 8)
-            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 8, 1, true>, grid, dim3(512), p);
-        else
-            launch(e, ljk::k_pair_cells_mixed<LAW, 2, 4, 1, true>, grid, dim3(256), p);
-    } else {
-        launch(e, ljk::k_pair_cells_fp64<LAW, 2, 1, true>, grid, dim3(64 * ljk::kCellsPerBlockFp64), p);
-    }
+    if (e->precision == LJ_PRECISION_MIXED) launch_tiled_shape<ljk::MIXED, LAW, 2, true>(e, p, grid);
+    else launch_tiled_shape<ljk::FP64, LAW, 2, true>(e, p, grid);
 }
 
 // One resident step of parity r: the pair kernel with the drift and the

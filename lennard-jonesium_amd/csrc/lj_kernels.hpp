@@ -1492,7 +1492,7 @@ __constant__ const SliceGeomTable kSliceGeom = SliceGeomTable();
 // waves, wave sub taking the neighbour cells q with q % NWC == sub; their
 // partial forces are added in wave order before the kick.  The grid then runs
 // cpb workgroups per cell-space block.
-template <int LAW, int KICK, int NW, int NWC = 1, bool RES = false>
+template <int LAW, int KICK, int NW, int NWC = 1, bool RES = false, bool UNIT = false>
 __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs p)
 {
     static_assert(NW == 1 || NWC == 1, "several waves per cell: one cell per block");
@@ -1729,7 +1729,7 @@ __global__ __launch_bounds__(64 * NW * NWC, 6) void k_pair_cells_mixed(StepArgs 
                         const float dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                         const float inv = __builtin_amdgcn_rcpf(r2);
-                        const float t = inv * inv_s2;
+                        const float t = UNIT ? inv : inv * inv_s2;   // UNIT: sigma = 1, a multiply by 1
                         const float s6 = t * t * t;
                         const float g = s6 * __builtin_fmaf(2.0f, s6, -1.0f);
                         gs += g;
@@ -1895,7 +1895,7 @@ __constant__ const SliceGeomTableD kSliceGeomD = SliceGeomTableD();
 // neighbour cells q with q % NWC == sub; their partial forces are added in
 // wave order before the kick.  The system's flag names the kernel: tiled 1
 // for NWC = 1, 2 for NWC > 1.
-template <int LAW, int KICK, int NWC = 1, bool RES = false>
+template <int LAW, int KICK, int NWC = 1, bool RES = false, bool UNIT = false>
 __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells_fp64(StepArgs p)
 {
     static_assert(!RES || KICK == 2, "resident cells: chained steps");
@@ -2145,7 +2145,7 @@ __global__ __launch_bounds__(64 * kCellsPerBlockFp64 * NWC, 4) void k_pair_cells
                             // accumulated as g / 24 = s (2 s - 1) and F / 24 = (g / 24) / r^2 +
                             // cw / 24 (the 24 is restored per particle)
                             const double inv = rcp_f64(r2);
-                            const double t = inv * sig2;
+                            const double t = UNIT ? inv : inv * sig2;   // UNIT: sigma = 1, a multiply by 1
                             const double s6 = t * t * t;
                             const double g = s6 * __builtin_fma(2.0, s6, -1.0);
                             ga += g;

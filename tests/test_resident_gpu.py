@@ -138,3 +138,33 @@ def test_resident_fallback_when_cells_overflow():
     b = _run(eng, systems, "mixed", [20, 20], False)
     assert a[2] == (2, 2), f"resident calls / fallbacks: {a[2]}"
     _same(a, b)
+
+
+def test_resident_noalloc_after_allocation():
+    """lj_engine_step_noalloc uses resident cells once an allocating call has
+    created their buffers: its records and state equal lj_engine_step's."""
+    import ctypes
+
+    from lennardjonesium import _native
+
+    eng = _eng()
+    systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5)])
+    _say("resident noalloc")
+    es = []
+    for _ in range(2):
+        e = eng.Engine([s for s, _, _ in systems], precision="mixed")
+        e.set_state(0, systems[0][1], systems[0][2])
+        e.step(8)   # allocates the resident buffers
+        es.append(e)
+    a, b = es
+    want = a.step(40)
+    got = np.full((40, 1, 7), np.nan)
+    _native.check(_native.engine().lj_engine_step_noalloc(
+        b.h, 40, got.ctypes.data_as(ctypes.POINTER(_native.lj_thermo))))
+    assert np.array_equal(got, want)
+    sa, sb = a.get_state(0), b.get_state(0)
+    for f in FIELDS:
+        assert np.array_equal(sa[f], sb[f]), f
+    assert a.resident_stats() == (2, 0) and b.resident_stats() == (2, 0)
+    a.close()
+    b.close()
