@@ -192,9 +192,17 @@ struct lj_engine {
     int *r_base = nullptr, *r_count = nullptr, *r_end = nullptr, *r_dirty = nullptr;
     int *r_narr = nullptr, *r_flag = nullptr, *d_cell_sys = nullptr;
     ljk::ResMover* r_arr = nullptr;
-    double* r_ck = nullptr;            // a call's starting forces, time, PE, W (for a fallback)
+    double* r_force = nullptr;         // forces of the last step of a call that ended resident
+    double* r_ck = nullptr;            // time, PE, W of the snapshot (for a fallback)
     hipGraphExec_t rgraphs[2][5] = {}; // resident steps: 2^(l+1) steps from parity s
     long long res_calls = 0, res_fallbacks = 0;
+    // Live resident cells: the resident buffers of parity res_par hold the
+    // current state (the last call ended there, its forces in r_force).  The
+    // packed set `cur` and d_force then still hold the snapshot the resident
+    // calls started from, res_since steps earlier, with its clocks in r_ck.
+    bool res_live = false;
+    int res_par = 0;
+    long long res_since = 0;
 
     bool profiling = false;
     std::vector<KernelTimer> timers;
@@ -278,6 +286,7 @@ struct lj_engine {
         p.id = p.id_n = r_id[0];
         p.lpos = r_lpos[a];
         p.lpos_n = r_lpos[r ^ 1];
+        p.force = planes(r_force);
         p.start = const_cast<int*>(r_base);
         p.cend = r_end;
         p.pending = d_pending + (size_t)r * nsys;
@@ -841,6 +850,7 @@ int check_sys(const lj_engine* e, int sys)
 int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records);
 bool res_eligible(const lj_engine* e, int nsteps);
 int res_ensure(lj_engine* e);
+int res_settle(lj_engine* e);
 int build_res_graphs(lj_engine* e);
 
 // Ensures slots are in cell order without moving particles (needed before
@@ -1126,7 +1136,7 @@ int lj_engine_destroy(lj_engine* e)
             if (m) (void)hipFree(m);
     }
     void* res[] = {e->r_base, e->r_count, e->r_end, e->r_dirty, e->r_narr,
-                   e->r_arr, e->r_flag, e->d_cell_sys, e->r_ck};
+                   e->r_arr, e->r_flag, e->d_cell_sys, e->r_ck, e->r_force};
     for (void* b : res)
         if (b) (void)hipFree(b);
     for (hipEvent_t ev : e->prof_ev) (void)hipEventDestroy(ev);
@@ -1144,6 +1154,7 @@ int lj_engine_set_state(lj_engine* e, int sys, const double* pos, const double* 
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (!pos || !vel) return fail(LJ_E_INVALID, "pos and vel are required");
     const SysDev& S = e->sys[sys];
     const int n = S.n;
@@ -1182,6 +1193,7 @@ int lj_engine_get_state(lj_engine* e, int sys, double* pos, double* vel, double*
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     const SysDev& S = e->sys[sys];
     const int n = S.n;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1262,17 +1274,21 @@ int lj_engine_reserve(lj_engine* e, int max_steps)
 namespace {
 
 // ---------------------------------------------------------------------------
-// Resident cells (lj_kernels.hpp section 4b).  A call of n >= kResMinSteps
-// steps on an engine whose systems all use the tiled kernel (all with one
-// wave per cell, or all with several) runs
-//   step 1:      key pass -> k_res_enter (into the resident slots) -> pair -> drift -> fix-up
-//   steps 2..n-1: pair -> drift -> fix-up (replayed from graphs of resident steps)
-//   step n:      scan of the resident counts -> k_res_leave (packed set) -> pair
-// The pair kernels see the same cells in the same order as on the packed
-// path, so records and state are the same bit for bit.  The packed set the
-// call started from is not written, so when a capacity was exceeded the call
-// is redone on the packed path from there (the forces, time, PE and W are
-// checkpointed at its start).
+// Resident cells (lj_kernels.hpp section 4b).  On an engine whose systems all
+// use the tiled kernel (all with one wave per cell, or all with several),
+// step calls run on resident cells and stay there between calls:
+//   entering (a call of n >= kResMinSteps steps from the packed set):
+//                key pass -> k_res_enter (the first cell build, into the resident slots)
+//   entering a call on live cells: k_res_kick_drift (stored forces) -> fix-up
+//   steps 1..n-1: pair with the drift -> fix-up (replayed from graphs)
+//   step n:       the pair kernel of a call's last step on the resident
+//                 layout (forces into r_force), then the record
+// Any other entry point first packs the live state back (res_settle: a scan of
+// the resident counts, k_res_leave with the forces).  The pair kernels see the
+// same cells in the same order as on the packed path, so records and state are
+// the same bit for bit, however the steps are split into calls.  The packed
+// set is not written while cells are live, so when a capacity was exceeded the
+// steps since it are redone on the packed path (its clocks are checkpointed).
 // ---------------------------------------------------------------------------
 constexpr int kResMinSteps = 4;
 // LJ_RESIDENT_AUTO: particles per cell at which resident cells are used.
@@ -1326,13 +1342,13 @@ int res_ensure(lj_engine* e)
             (e->precision == LJ_PRECISION_MIXED && (rc = dev_alloc(&e->r_lpos[b], ns))))
             return rc;
     if ((rc = dev_alloc(&e->r_vel[0], 3 * ns)) || (rc = dev_alloc(&e->r_disp[0], 3 * ns)) ||
-        (rc = dev_alloc(&e->r_id[0], ns)))
+        (rc = dev_alloc(&e->r_force, 3 * ns)) || (rc = dev_alloc(&e->r_id[0], ns)))
         return rc;
     if ((rc = dev_alloc(&e->r_base, nc)) || (rc = dev_alloc(&e->r_count, nc)) ||
         (rc = dev_alloc(&e->r_end, nc)) || (rc = dev_alloc(&e->r_dirty, nc)) ||
         (rc = dev_alloc(&e->r_narr, nc)) || (rc = dev_alloc(&e->r_arr, (size_t)nc * ljk::kResArr)) ||
         (rc = dev_alloc(&e->r_flag, 1)) || (rc = dev_alloc(&e->d_cell_sys, nc)) ||
-        (rc = dev_alloc(&e->r_ck, 3 * (size_t)e->nslot + 3 * (size_t)e->nsys)))
+        (rc = dev_alloc(&e->r_ck, 3 * (size_t)e->nsys)))
         return rc;
     // every slot starts as zeros (finite values wherever a kernel may read)
     for (int b = 0; b < 2; ++b) {
@@ -1341,6 +1357,7 @@ int res_ensure(lj_engine* e)
     }
     HIP_TRY(hipMemset(e->r_vel[0], 0, 3 * ns * sizeof(double)));
     HIP_TRY(hipMemset(e->r_disp[0], 0, 3 * ns * sizeof(double)));
+    HIP_TRY(hipMemset(e->r_force, 0, 3 * ns * sizeof(double)));
     HIP_TRY(hipMemset(e->r_id[0], 0, ns * sizeof(int)));
     HIP_TRY(hipMemcpy(e->r_base, base.data(), nc * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->r_end, base.data(), nc * sizeof(int), hipMemcpyHostToDevice));
@@ -1372,6 +1389,25 @@ void launch_res_pair(lj_engine* e, const ljk::StepArgs& p)
     }
     if (e->precision == LJ_PRECISION_MIXED) launch_tiled_shape<ljk::MIXED, LAW, 2, true>(e, p, grid);
     else launch_tiled_shape<ljk::FP64, LAW, 2, true>(e, p, grid);
+}
+
+// The pair kernel of a call's last step on the resident layout: the tiled
+// kernels of the packed path's last step (KICK 1: second half kick, forces
+// stored), with the resident cells' ranges and p.force = r_force.
+template <int LAW>
+void launch_res_last(lj_engine* e, const ljk::StepArgs& p)
+{
+    const dim3 grid(e->ncblk);
+    if (e->n_multi == e->nsys) {
+        const dim3 gm(e->ncblk * p.cpb), bm(64 * ljk::kWavesPerCellSmall);
+        if (e->precision == LJ_PRECISION_MIXED)
+            launch(e, ljk::k_pair_cells_mixed<LAW, 1, 1, ljk::kWavesPerCellSmall, false>, gm, bm, p);
+        else
+            launch(e, ljk::k_pair_cells_fp64<LAW, 1, ljk::kWavesPerCellSmall, false>, gm, bm, p);
+        return;
+    }
+    if (e->precision == LJ_PRECISION_MIXED) launch_tiled_shape<ljk::MIXED, LAW, 1, false>(e, p, grid);
+    else launch_tiled_shape<ljk::FP64, LAW, 1, false>(e, p, grid);
 }
 
 // One resident step of parity r: the pair kernel with the drift and the
@@ -1433,34 +1469,86 @@ int build_res_graphs(lj_engine* e)
     return rc;
 }
 
+// Packs live resident cells back into the other packed set (with the last
+// step's forces) and makes it current; nothing to do when none are live.
+int res_settle(lj_engine* e)
+{
+    if (!e->res_live) return LJ_OK;
+    int rc;
+    const int nsys = e->nsys;
+    ljk::StepArgs p = e->args(e->cur ^ 1);
+    p.count = e->r_count;
+    p.count_init = e->r_count;   // the two-level scan's count reset leaves them as they are
+    const ljk::FinalArgs f = e->fargs(0);
+    (void)nsys;
+    if (!e->two_level_scan()) {
+        if ((rc = timed(e, 1, "scan", [&] {
+                 launch(e, ljk::k_scan_one, dim3(1), dim3(ljk::kScanOneThreads), p, f);
+             })))
+            return rc;
+    } else {
+        if ((rc = timed(e, 1, "scan_tiles", [&] {
+                 launch(e, ljk::k_scan_tiles, dim3(e->ntile), dim3(kBlock), p);
+             })))
+            return rc;
+        if ((rc = timed(e, 2, "scan_apply", [&] {
+                 launch(e, ljk::k_scan_apply, dim3(e->ntile), dim3(kBlock), p, f);
+             })))
+            return rc;
+    }
+    ljk::StepArgs q = e->args_res(e->res_par);   // the live state
+    const ljk::StepArgs pk = e->args(e->cur ^ 1);
+    q.pos_n = pk.pos;
+    q.vel_n = pk.vel;
+    q.disp_n = pk.disp;
+    q.id_n = pk.id;
+    q.lpos_n = e->d_lpos;
+    q.start = e->d_start;
+    const ljk::Planes fdst = pk.force;
+    const dim3 grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
+    if ((rc = timed(e, 11, "res_leave", [&] {
+             if (e->precision == LJ_PRECISION_MIXED)
+                 launch(e, ljk::k_res_leave<true, true>, grid, dim3(kBlock), q, fdst);
+             else
+                 launch(e, ljk::k_res_leave<false, true>, grid, dim3(kBlock), q, fdst);
+         })))
+        return rc;
+    HIP_TRY(hipGetLastError());
+    e->cur ^= 1;
+    e->sorted = true;
+    e->prekicked = false;
+    e->res_live = false;
+    e->res_since = 0;
+    return LJ_OK;
+}
+
 // The step loop on resident cells (see above).  Returns 1 (with nothing to
-// report) when a capacity was exceeded and the caller must redo the call on
-// the packed path; the engine is then back at the call's starting state.
+// report) when a capacity was exceeded: the engine is then back at the
+// snapshot, and the caller redoes on the packed path the `res_since` steps
+// taken since it and this call's steps.
 int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
 {
     int rc;
-    const int cur0 = e->cur;
-    const bool sorted0 = e->sorted;
     const bool mixed = e->precision == LJ_PRECISION_MIXED;
-    const size_t ns = e->nslot;
     const int nsys = e->nsys;
     e->res_calls++;
     if (graphs && (rc = build_res_graphs(e))) return rc;
-    // checkpoint of what the call overwrites outside the starting buffer set
-    HIP_TRY(hipMemcpyAsync(e->r_ck, e->d_force, 3 * ns * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns, e->d_time, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns + nsys, e->d_pe, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->r_ck + 3 * ns + 2 * nsys, e->d_vir, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemsetAsync(e->r_flag, 0, sizeof(int), e->stream));
     const ljk::FinalArgs f = e->fargs(0);
     const long long nrec = (long long)nsteps * nsys * 7;
     const long long nthr = std::max<long long>(nrec, nsys);
     hipLaunchKernelGGL(ljk::k_begin_call,
                        dim3((unsigned)std::min<long long>(4096, (nthr + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, e->stream, f.records, nrec, e->d_step, nsys);
-    // step 1: key pass of the integrated positions, then into the resident slots
-    {
-        const ljk::StepArgs p = e->args(cur0);
+    const dim3 cell_grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
+    int r;
+    if (!e->res_live) {
+        // the snapshot: the packed set e->cur and d_force stay as they are;
+        // the clocks are checkpointed
+        hipLaunchKernelGGL(ljk::k_res_checkpoint, dim3((nsys + kBlock - 1) / kBlock), dim3(kBlock),
+                           0, e->stream, f, e->r_ck);
+        e->res_since = 0;
+        // the first half kick and drift: key pass, then into the resident slots
+        const ljk::StepArgs p = e->args(e->cur);
         if ((rc = timed(e, 0, "integrate_key", [&] {
                  launch(e, ljk::k_integrate_key<2, true>, dim3(e->nblk), dim3(kBlock), p, f, 0);
              })))
@@ -1473,19 +1561,39 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
         q.id_n = r0.id;
         q.lpos_n = r0.lpos;
         q.rc_par = 0;
-        const dim3 grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
         if ((rc = timed(e, 10, "res_enter", [&] {
-                 if (mixed) launch(e, ljk::k_res_enter<true, 1>, grid, dim3(kBlock), q);
-                 else launch(e, ljk::k_res_enter<false, 1>, grid, dim3(kBlock), q);
+                 if (mixed) launch(e, ljk::k_res_enter<true, 1>, cell_grid, dim3(kBlock), q);
+                 else launch(e, ljk::k_res_enter<false, 1>, cell_grid, dim3(kBlock), q);
              })))
             return rc;
+        r = 0;
+    } else {
+        // the first half kick and drift from the stored forces, then the fix-up
+        r = e->res_par;
+        const ljk::StepArgs p = e->args_res(r);
+        if ((rc = timed(e, 12, "res_drift", [&] {
+                 if (mixed) launch(e, ljk::k_res_kick_drift<true>, cell_grid, dim3(kBlock), p);
+                 else launch(e, ljk::k_res_kick_drift<false>, cell_grid, dim3(kBlock), p);
+             })))
+            return rc;
+        const ljk::StepArgs q = e->args_res(r, true);
+        const ljk::FinalArgs f1 = e->fargs_par(r), f2 = e->fargs_par(r ^ 1);
+        if ((rc = timed(e, 9, "fixup", [&] {
+                 if (mixed) launch(e, ljk::k_res_fixup<true>, dim3(e->res_nfix), dim3(kBlock), q, f1, f2, 0, 0);
+                 else launch(e, ljk::k_res_fixup<false>, dim3(e->res_nfix), dim3(kBlock), q, f1, f2, 0, 0);
+             })))
+            return rc;
+        r ^= 1;
     }
-    int r = 0;
-    if ((rc = enqueue_res_step(e, r, true))) return rc;
-    r ^= 1;
-    // steps 2 .. n - 1
-    int done = 1;
+    e->res_live = true;
+    // steps 1 .. n - 1: pair kernel with the drift, fix-up
     const int middle_end = nsteps - 1;
+    int done = 0;
+    if (middle_end > 0) {
+        if ((rc = enqueue_res_step(e, r, true))) return rc;
+        r ^= 1;
+        done = 1;
+    }
     if (graphs) {
         for (int l = kGraphLevels - 1; l >= 0; --l) {
             const int len = 2 << l;
@@ -1497,48 +1605,27 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     }
     for (; done < middle_end; ++done, r ^= 1)
         if ((rc = enqueue_res_step(e, r, false))) return rc;
-    // step n: pack the resident slots into the other packed set (the scan's
-    // extra blocks finish step n - 1's record), then the pair kernel as usual
+    // step n: forces into r_force and the second half kick, then the records
+    // (the second level of step n - 1's, both levels of step n's)
     {
-        ljk::StepArgs p = e->args(cur0 ^ 1);
-        p.count = e->r_count;
-        p.count_init = e->r_count;   // the two-level scan's count reset leaves them as they are
-        const ljk::FinalArgs fl = e->fargs_par(r ^ 1);
-        if (!e->two_level_scan()) {
-            if ((rc = timed(e, 1, "scan", [&] {
-                     launch(e, ljk::k_scan_one, dim3(1 + nsys), dim3(ljk::kScanOneThreads), p, fl);
-                 })))
-                return rc;
-        } else {
-            if ((rc = timed(e, 1, "scan_tiles", [&] {
-                     launch(e, ljk::k_scan_tiles, dim3(e->ntile), dim3(kBlock), p);
-                 })))
-                return rc;
-            if ((rc = timed(e, 2, "scan_apply", [&] {
-                     launch(e, ljk::k_scan_apply, dim3(e->ntile + nsys), dim3(kBlock), p, fl);
-                 })))
-                return rc;
-        }
-        ljk::StepArgs q = e->args_res(r);   // the state of parity r
-        const ljk::StepArgs pk = e->args(cur0 ^ 1);
-        q.pos_n = pk.pos;
-        q.vel_n = pk.vel;
-        q.disp_n = pk.disp;
-        q.id_n = pk.id;
-        q.lpos_n = e->d_lpos;
-        q.start = e->d_start;
-        const dim3 grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
-        if ((rc = timed(e, 11, "res_leave", [&] {
-                 if (mixed) launch(e, ljk::k_res_leave<true>, grid, dim3(kBlock), q);
-                 else launch(e, ljk::k_res_leave<false>, grid, dim3(kBlock), q);
+        const ljk::StepArgs p = e->args_res(r);
+        e->last_pair_force = true;
+        if ((rc = timed(e, 5, "pair", [&] {
+                 if (e->sys[0].law == ljk::LAW_LJ) launch_res_last<ljk::LAW_LJ>(e, p);
+                 else launch_res_last<ljk::LAW_CONSTANT>(e, p);
              })))
             return rc;
-        e->cur = cur0 ^ 1;
-        e->sorted = true;
-        e->prekicked = false;
-        if ((rc = enqueue_pair(e, 1, false))) return rc;
+        const ljk::FinalArgs f1 = e->fargs_par(r), f2 = e->fargs_par(r ^ 1);
+        const int nred = middle_end > 0 ? nsys : 0;
+        const int nfin = nsys * f1.nf;
+        if ((rc = timed(e, 6, "finalize", [&] {
+                 if (mixed) launch(e, ljk::k_res_fixup<true>, dim3(nred + nfin), dim3(kBlock), p, f1, f2, nred, nfin);
+                 else launch(e, ljk::k_res_fixup<false>, dim3(nred + nfin), dim3(kBlock), p, f1, f2, nred, nfin);
+                 launch(e, ljk::k_finalize_reduced, dim3(nsys), dim3(kBlock), f1);
+             })))
+            return rc;
     }
-    if ((rc = enqueue_finalize(e, 0))) return rc;
+    e->res_par = r;
     HIP_TRY(hipGetLastError());
     int flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, e->r_flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -1546,17 +1633,22 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
         HIP_TRY(hipMemcpyAsync(records, f.records, (size_t)nsteps * nsys * sizeof(lj_thermo),
                                hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if (!flag) return LJ_OK;
-    // a capacity was exceeded: back to the call's starting state
+    if (!flag) {
+        e->res_since += nsteps;
+        return LJ_OK;
+    }
+    // a capacity was exceeded: back to the snapshot (the packed set e->cur,
+    // d_force and the checkpointed clocks)
     e->res_fallbacks++;
-    e->cur = cur0;
-    e->sorted = sorted0;
+    e->res_live = false;
     e->prekicked = false;
-    HIP_TRY(hipMemcpyAsync(e->d_force, e->r_ck, 3 * ns * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_time, e->r_ck + 3 * ns, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_pe, e->r_ck + 3 * ns + nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_vir, e->r_ck + 3 * ns + 2 * nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_time, e->r_ck, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_pe, e->r_ck + nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_vir, e->r_ck + 2 * nsys, nsys * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     HIP_TRY(hipMemsetAsync(e->d_pending, 0, 2 * nsys * sizeof(int), e->stream));
+    // the entering key pass's overflow entries (cells beyond the member
+    // records), which no scan consumed
+    HIP_TRY(hipMemsetAsync(e->d_novf, 0, 2 * sizeof(int), e->stream));
     HIP_TRY(hipMemsetAsync(e->r_dirty, 0, e->ncell_total * sizeof(int), e->stream));
     HIP_TRY(hipMemsetAsync(e->r_narr, 0, e->ncell_total * sizeof(int), e->stream));
     HIP_TRY(hipMemsetAsync(e->r_flag, 0, sizeof(int), e->stream));
@@ -1564,19 +1656,38 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     return 1;
 }
 
+int run_steps_packed(lj_engine* e, int nsteps, bool graphs, lj_thermo* records);
+
 // The step loop shared by the step entry points: nsteps steps of every
 // system, records into the engine's current record buffer (fargs), replayed
 // from the captured step graphs when `graphs` is set, then copied to
-// `records` (optional).
+// `records` (optional).  On live resident cells, or entering them (allocating
+// callers always, the others once allocated); after a capacity was exceeded
+// there, the steps since the snapshot are redone on the packed path (in calls
+// of at most the record capacity, whose records are dropped: a run's results
+// do not depend on how it is split into calls), then this call's.
 int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
 {
     int rc;
-    // resident cells: allocating callers always, the others once allocated
-    if (res_eligible(e, nsteps) && (e->res_ready || !e->rec_buf)) {
+    if (e->res_live || (res_eligible(e, nsteps) && (e->res_ready || !e->rec_buf))) {
         if ((rc = res_ensure(e))) return rc;
+        const long long since = e->res_live ? e->res_since : 0;
         rc = run_steps_res(e, nsteps, graphs, records);
         if (rc != 1) return rc;
+        const int cap = std::max(1, e->fargs(0).cap);
+        for (long long left = since; left > 0;) {
+            const int n = (int)std::min<long long>(left, cap);
+            if ((rc = run_steps_packed(e, n, graphs, nullptr))) return rc;
+            left -= n;
+        }
     }
+    return run_steps_packed(e, nsteps, graphs, records);
+}
+
+// The packed path's step loop (a cell build every step).
+int run_steps_packed(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
+{
+    int rc;
     const ljk::FinalArgs f = e->fargs(0);
     const long long nrec = (long long)nsteps * e->nsys * 7;
     const long long nthr = std::max<long long>(nrec, e->nsys);
@@ -1617,6 +1728,7 @@ int lj_engine_compute_forces(lj_engine* e, double* pe, double* vir)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (e->has_cells) {
         if ((rc = enqueue_cells(e, false))) return rc;
         if ((rc = enqueue_pair(e, false, false))) return rc;
@@ -1633,6 +1745,7 @@ int lj_engine_measure(lj_engine* e, lj_thermo* out)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if ((rc = enqueue_pair(e, false, true))) return rc;
     if ((rc = enqueue_finalize(e, 2))) return rc;
     HIP_TRY(hipGetLastError());
@@ -1647,6 +1760,7 @@ int lj_engine_set_temperature(lj_engine* e, int sys, double target, double* meas
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     std::vector<lj_thermo> m(e->nsys);
     if ((rc = lj_engine_measure(e, m.data()))) return rc;
     const double t = m[sys].temperature;
@@ -1665,6 +1779,7 @@ int lj_engine_cell_keys(lj_engine* e, int sys, int* keys, int* nc, int* clamped)
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     const SysDev& S = e->sys[sys];
     if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
     if (nc)
@@ -1699,6 +1814,7 @@ int lj_engine_cell_list(lj_engine* e, int sys, int* start, int* atoms)
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     const SysDev& S = e->sys[sys];
     if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
     if ((rc = enqueue_cells(e, false))) return rc;
@@ -1719,6 +1835,7 @@ int lj_engine_pairs(lj_engine* e, int sys, long long cap, int* idx, double* sep,
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     const SysDev& S = e->sys[sys];
     if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
     if (cap < 0) return fail(LJ_E_INVALID, "cap < 0");
@@ -1759,6 +1876,7 @@ int lj_engine_neighbor_counts(lj_engine* e, int sys, int* counts)
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     const SysDev& S = e->sys[sys];
     if (S.law == LJ_LAW_NONE) return fail(LJ_E_STATE, "system has no cell grid");
     if ((rc = dev_alloc(&e->d_ncount, e->nslot))) return rc;
@@ -1786,6 +1904,7 @@ int lj_engine_rdf_sample(lj_engine* e, double rmax, int bins)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (!e->has_cells) return fail(LJ_E_STATE, "no cell grid");
     if (!(rmax > 0) || bins < 1 || bins > 4096) return fail(LJ_E_INVALID, "bad rmax / bins");
     if (e->d_rdf && (bins != e->rdf_bins || rmax != e->rdf_rmax))
@@ -1831,6 +1950,7 @@ int lj_engine_set_active(lj_engine* e, int sys, int active)
 {
     int rc = check_sys(e, sys);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->sys[sys].active = active ? 1 : 0;
     HIP_TRY(hipMemcpy(e->d_sys + sys, &e->sys[sys], sizeof(SysDev), hipMemcpyHostToDevice));
@@ -1849,6 +1969,7 @@ int lj_engine_set_resident_cells(lj_engine* e, int mode)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (mode != LJ_RESIDENT_OFF && mode != LJ_RESIDENT_AUTO && mode != LJ_RESIDENT_ON)
         return fail(LJ_E_INVALID, "resident cells: mode must be LJ_RESIDENT_OFF, _AUTO or _ON");
     e->res_mode = mode;
@@ -1872,6 +1993,7 @@ int lj_engine_set_pair_kernel(lj_engine* e, int variant)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (variant != LJ_PAIR_PER_PARTICLE && variant != LJ_PAIR_CELL_TILED &&
         variant != LJ_PAIR_BY_OCCUPANCY && variant != LJ_PAIR_CELL_MULTIWAVE)
         return fail(LJ_E_INVALID, "unknown pair kernel variant");
@@ -1883,6 +2005,7 @@ int lj_engine_set_pair_split(lj_engine* e, double min_occupancy)
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (!(min_occupancy > 0)) return fail(LJ_E_INVALID, "min_occupancy must be > 0");
     e->split_occupancy = min_occupancy;
     e->pair_variant = LJ_PAIR_BY_OCCUPANCY;
@@ -1893,6 +2016,7 @@ int lj_engine_set_launch_shape(lj_engine* e, int cells_per_block, int sort_wave_
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (cells_per_block != 2 && cells_per_block != 4 && cells_per_block != 8)
         return fail(LJ_E_INVALID, "cells_per_block must be 2, 4 or 8");
     if (sort_wave_particles < 8 || sort_wave_particles > 64)
@@ -1928,6 +2052,7 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (!e->has_cells) {
         if (chosen) *chosen = e->pair_variant;
         return LJ_OK;
@@ -1968,6 +2093,8 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
         e->cur = cur0;
         e->sorted = false;
         e->prekicked = false;
+        e->res_live = false;   // a trial's resident cells are dropped
+        e->res_since = 0;
         hipError_t h = cp(e->d_pos[cur0], sp.p, 3 * ns * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_vel[cur0], sv.p, 3 * ns * sizeof(double));
         if (h == hipSuccess) h = cp(e->d_disp[cur0], sd.p, 3 * ns * sizeof(double));
@@ -2020,12 +2147,12 @@ int lj_engine_autotune(lj_engine* e, int steps, int* chosen, double* ms_per_step
     }
     // One candidate: calls of `steps` and of 2 `steps` steps, timed with
     // events; their difference over `steps` is the time per step of long
-    // calls (what a call costs once, such as a resident call's cell build and
-    // packing, cancels out).
+    // calls (what a call costs once cancels out).
     auto trial = [&](double* per_step) -> int {
         int trc;
         if ((trc = lj_engine_reserve(e, 2 * steps))) return trc;   // graphs built untimed
-        if ((trc = lj_engine_step(e, 2, nullptr))) return trc;     // warm-up
+        // warm-up (resident trials: the timed calls then both start on live cells)
+        if ((trc = lj_engine_step(e, kResMinSteps, nullptr))) return trc;
         float ms[2] = {0, 0};
         for (int r = 0; r < 2; ++r) {
             hipError_t h = hipEventRecord(e->ev0, e->stream);
@@ -2139,6 +2266,7 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
 {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = res_settle(e))) return rc;
     if (e->nsys != 1 || e->precision != LJ_PRECISION_FP64 || e->sys[0].law != LJ_LAW_LJ)
         return fail(LJ_E_INVALID, "the adjoint needs a single-system FP64 LJ engine");
     if (nsteps < 1 || every < 1) return fail(LJ_E_INVALID, "nsteps and checkpoint interval must be >= 1");
@@ -2201,6 +2329,7 @@ extern "C" int lj_engine_adjoint(lj_engine* e, int nsteps, int every, double den
     for (int t = 0; t < nsteps;) {
         const int k = std::min(every, nsteps - t);
         if ((rc = lj_engine_step(e, k, rec.data() + t))) return rc;
+        if ((rc = res_settle(e))) return rc;   // the checkpoint reads the packed set
         t += k;
         if (t % every == 0) to_id(ck_pos.p + (size_t)(t / every) * n3, ck_vel.p + (size_t)(t / every) * n3);
     }

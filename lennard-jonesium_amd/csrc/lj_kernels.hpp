@@ -836,12 +836,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_cells(StepArgs p)
 //   (neighbours read the current one while owners write the next);
 //   velocities, displacements and ids are updated in place.  The pair kernels therefore see the same candidates in the
 //   same order as after a full cell build, and results are the same bit for
-//   bit.  k_res_enter moves a call's first cell build into the resident
-//   slots; k_res_leave packs the state back (after a scan of rc_count) for
-//   the call's last step.  Capacities exceeded anywhere (a cell beyond
-//   kResCap, more than kResArr arrivals in a step) set rc_flag; every kernel
-//   stays within its buffers regardless, and the host then redoes the call
-//   on the packed path from the state the call started from.
+//   bit.  k_res_enter moves a first cell build into the resident slots; a
+//   call's last step stores its forces in the resident slots, and the next
+//   call starts with k_res_kick_drift; k_res_leave packs the state back (after
+//   a scan of rc_count) when anything else needs the packed layout.
+//   Capacities exceeded anywhere (a cell beyond kResCap, more than kResArr
+//   arrivals in a step) set rc_flag; every kernel stays within its buffers
+//   regardless, and the host then redoes the steps on the packed path from
+//   the packed state the resident steps started from.
 // ---------------------------------------------------------------------------
 
 // The first cell build of a call into the resident slots: one wave per cell takes
@@ -1000,8 +1002,10 @@ __global__ __launch_bounds__(kBlock) void k_res_fixup(StepArgs p, FinalArgs f1, 
 // Packs resident set p.pos etc. into the packed layout of set p.pos_n etc.:
 // cell c's members go to start[c] + rank (start: the scan of rc_count), with
 // their cell-local coordinates and cells, as a sort leaves them.
-template <bool MIXED_>
-__global__ __launch_bounds__(kBlock) void k_res_leave(StepArgs p)
+// FORCE: also the resident forces p.force into fdst (a call that ended on
+// resident cells stored its last step's forces there).
+template <bool MIXED_, bool FORCE>
+__global__ __launch_bounds__(kBlock) void k_res_leave(StepArgs p, Planes fdst)
 {
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1018,6 +1022,10 @@ __global__ __launch_bounds__(kBlock) void k_res_leave(StepArgs p)
     p.id_n[dst] = p.id[src];
     p.scell[dst] = c;
     if (MIXED_) p.lpos_n[dst] = p.lpos[src];
+    if (FORCE) {
+        fdst.x[dst] = p.force.x[src]; fdst.y[dst] = p.force.y[src];
+        fdst.z[dst] = p.force.z[src];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,6 +1189,11 @@ __device__ __forceinline__ void kick_store(const StepArgs& p, const SysDev& S, i
 // as an arrival record of its new cell (its slot is vacated: id -1); both
 // cells are marked.  d: the displacement before the drift.
 template <bool MIXED_>
+__device__ __forceinline__ void res_place(const StepArgs& p, const SysDev& S, int c, int si,
+                                          double x, double y, double z, double wx, double wy,
+                                          double wz, double nx, double ny, double nz);
+
+template <bool MIXED_>
 __device__ __forceinline__ void res_kick_drift(const StepArgs& p, const SysDev& S, int c, int si,
                                                double fx, double fy, double fz, double& vx,
                                                double& vy, double& vz, double ddx, double ddy,
@@ -1196,7 +1209,20 @@ __device__ __forceinline__ void res_kick_drift(const StepArgs& p, const SysDev& 
     double x = p.pos.x[si], y = p.pos.y[si], z = p.pos.z[si];
     double ix, iy, iz;
     advance<false>(S, x, y, z, wx, wy, wz, 0., 0., 0., ix, iy, iz);
-    const double nx = ddx + ix, ny = ddy + iy, nz = ddz + iz;
+    res_place<MIXED_>(p, S, c, si, x, y, z, wx, wy, wz, ddx + ix, ddy + iy, ddz + iz);
+}
+
+// Slot si of cell c after its drift: the new (wrapped) position x, y, z, the
+// velocity w carried into the next step and the displacement n.  A particle
+// that stays in c keeps its slot (position and cell-local coordinates into
+// the next sets, velocity and displacement in place); one that left becomes
+// an arrival record of its new cell (the slot is vacated: id -1), and both
+// cells are marked for the fix-up.
+template <bool MIXED_>
+__device__ __forceinline__ void res_place(const StepArgs& p, const SysDev& S, int c, int si,
+                                          double x, double y, double z, double wx, double wy,
+                                          double wz, double nx, double ny, double nz)
+{
     const int kx = cell_axis(x, S.q[0], S.nc[0]);
     const int ky = cell_axis(y, S.q[1], S.nc[1]);
     const int kz = cell_axis(z, S.q[2], S.nc[2]);
@@ -1224,6 +1250,29 @@ __device__ __forceinline__ void res_kick_drift(const StepArgs& p, const SysDev& 
     p.id[si] = -1;   // the slot is vacated
     res_mark(p, c);
     res_mark(p, cn);
+}
+
+// The first half kick and drift of a call that starts on live resident cells
+// (the previous call's last step stored its forces in the resident slots):
+// one wave per cell, lane t for slot rc_base[c] + t, with the arithmetic of
+// the key pass and the sort (advance), then res_place.  The fix-up follows.
+template <bool MIXED_>
+__global__ __launch_bounds__(kBlock) void k_res_kick_drift(StepArgs p)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (c >= p.ncell_total) return;
+    const SysDev& S = p.sys[p.cell_sys[c]];
+    if (c - S.cell_off >= S.ncell) return;   // tail cell: padding slots
+    if (lane >= p.rc_count[c]) return;
+    const int si = p.rc_base[c] + lane;
+    double x = p.pos.x[si], y = p.pos.y[si], z = p.pos.z[si];
+    double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+    const double fx = p.force.x[si], fy = p.force.y[si], fz = p.force.z[si];
+    const double dx = p.disp.x[si], dy = p.disp.y[si], dz = p.disp.z[si];
+    double ix, iy, iz;
+    advance<true>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+    res_place<MIXED_>(p, S, c, si, x, y, z, vx, vy, vz, dx + ix, dy + iy, dz + iz);
 }
 
 
@@ -2368,6 +2417,17 @@ __global__ __launch_bounds__(kBlock) void k_begin_call(double* v, long long n, i
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long k = t0; k < n; k += stride) v[k] = 0.0;
     for (long long k = t0; k < nsys; k += stride) step[k] = 0;
+}
+
+// Copies every system's time, PE and W (n each) to ck: the clocks of a
+// resident snapshot, put back if a later call must be redone.
+__global__ __launch_bounds__(kBlock) void k_res_checkpoint(FinalArgs f, double* ck)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= f.nsys) return;
+    ck[k] = f.time[k];
+    ck[f.nsys + k] = f.pe[k];
+    ck[2 * f.nsys + k] = f.vir[k];
 }
 
 // v *= scale for the slots of one system

@@ -1,8 +1,9 @@
 """Resident cells: step calls that keep every cell in a fixed block of slots and
-move only the particles that change cell must leave records and state equal,
-bit for bit, to the same calls with the full cell build every step (the
-reference's per-step cell lists, engine/particle_pair_filter.cpp:141-156).
-Calls that exceed a capacity are redone with the full build: same results."""
+move only the particles that change cell (and stay on them between calls)
+must leave records and state equal, bit for bit, to the same calls with the
+full cell build every step (the reference's per-step cell lists,
+engine/particle_pair_filter.cpp:141-156).  Steps that exceed a capacity are
+redone with the full build: same results."""
 
 import sys
 
@@ -87,17 +88,78 @@ def test_resident_small_systems_multiwave(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_resident_split_calls_and_eager(precision):
-    """Calls of 5, 37, 4, 3 (below the resident minimum: full build), 2 and
-    100 steps, replayed from graphs and launched eagerly, equal one call of
-    151 steps with the full build."""
+    """Calls of 5, 37, 4, 3, 2 and 100 steps (the first enters resident
+    cells, the others continue on them), replayed from graphs and launched
+    eagerly, equal one call of 151 steps with the full build."""
     eng = _eng()
     systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5)])
     _say("resident split calls,", precision)
     ref = _run(eng, systems, precision, [151], False)
     for graphs in (True, False):
         a = _run(eng, systems, precision, [5, 37, 4, 3, 2, 100], True, graphs=graphs)
-        assert a[2] == (4, 0), f"resident calls / fallbacks: {a[2]}"
+        assert a[2] == (6, 0), f"resident calls / fallbacks: {a[2]}"
         _same(a, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_live_between_calls(precision):
+    """Between resident calls: reading the state, measuring, a velocity
+    rescale and a call of 1 step.  Each reads or changes the packed state
+    (the live cells are packed back first) and the steps after it continue
+    exactly as on the full-build path."""
+    eng = _eng()
+    systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5), (4000, 0.8, 1.0, 11, 2.5)])
+    _say("resident live between calls,", precision)
+    out = []
+    for resident in (True, False):
+        e = eng.Engine([s for s, _, _ in systems], precision=precision)
+        e.set_resident_cells(resident)
+        for k, (_, p, v) in enumerate(systems):
+            e.set_state(k, p, v)
+        got = [e.step(7)]
+        got.append(e.get_state(1))
+        got.append(e.step(1))
+        got.append(e.step(9))
+        got.append(e.measure())
+        got.append(e.step(4))
+        e.set_temperature(0, 0.9)
+        got.append(e.step(12))
+        got.append(e.get_state(0))
+        got.append(e.get_state(1))
+        got.append(e.resident_stats())
+        e.close()
+        out.append(got)
+    a, b = out
+    assert a[-1] == (4, 0) and b[-1] == (0, 0), (a[-1], b[-1])
+    for x, y in zip(a[:-1], b[:-1]):
+        if isinstance(x, dict):
+            for f in FIELDS:
+                assert np.array_equal(x[f], y[f]), f
+            assert (x["time"], x["pe"], x["vir"]) == (y["time"], y["pe"], y["vir"])
+        else:
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_fallback_after_live_calls(precision):
+    """Nine particles on a 3 x 3 grid at one x, moving along +x at the same
+    speed, all enter the next cell in the sixth step: more arrivals than one
+    cell takes in a step.  Calls of 4, 4 and 4 steps: the second call, on live
+    cells, is redone with the previous call's steps on the full-build path;
+    the third enters resident cells again.  Records and state equal the
+    full-build run's."""
+    eng = _eng()
+    L = 7.8   # 3 cells of 2.6 per axis; the crossing is at x = 2.6
+    ys = (2.8, 3.9, 5.0)
+    pos = np.array([[2.545, y, z] for y in ys for z in ys])
+    vel = np.tile([2.0, 0.0, 0.0], (9, 1))
+    systems = [(eng.SystemSpec(9, L), pos, vel)]
+    _say("resident fallback after live calls,", precision)
+    a = _run(eng, systems, precision, [4, 4, 4], True)
+    b = _run(eng, systems, precision, [4, 4, 4], False)
+    assert a[2] == (3, 1), f"resident calls / fallbacks: {a[2]}"
+    _same(a, b)
+    assert a[1][0]["pos"][0, 0] > 2.6   # the crossing happened
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
