@@ -75,6 +75,11 @@ struct KernelTimer {
 // shared by all engines of that device and kept until the process ends.
 constexpr int kScratchRows = LJ_NOALLOC_MAX_ROWS;
 
+// Record doubles a step call hands back through its engine's pinned buffer
+// (about 2,300 rows: a 20-step call of up to 117 systems); longer record
+// sets are copied to the host directly.
+constexpr long long kHostOut = 16384;
+
 }  // namespace
 
 static __device__ __align__(256) double scratch[kScratchRows * 7];
@@ -164,6 +169,10 @@ struct lj_engine {
     int rdf_bins = 0, rdf_samples = 0;
     double rdf_rmax = 0;
 
+    // Pinned host memory for a call's records and flag (kHostOut doubles and
+    // one more), written by k_records_out through its device address h_out_d
+    double* h_out = nullptr;
+    double* h_out_d = nullptr;
     int* d_ncount = nullptr;    // set only while a neighbour-count hook runs
     int cur = 0;                // which buffer set holds the current state
     bool sorted = false;        // slots are in cell order (after a cell build)
@@ -628,6 +637,32 @@ int enqueue_finalize(lj_engine* e, int mode)
     });
 }
 
+// The end of a step call: copies its nsteps record rows to `records`
+// (optional) and the resident overflow flag to *flag (optional), then waits
+// for the stream.  Up to kHostOut doubles go through one k_records_out launch
+// into the pinned buffer; more through copies to the host.
+int finish_call(lj_engine* e, int nsteps, lj_thermo* records, int* flag)
+{
+    const double* rec = e->fargs(0).records;
+    const long long n = records ? (long long)nsteps * e->nsys * 7 : 0;
+    if (n <= kHostOut) {
+        if (records || flag) {
+            const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(64, (n + kBlock - 1) / kBlock));
+            hipLaunchKernelGGL(ljk::k_records_out, dim3(grid), dim3(kBlock), 0, e->stream, rec, n,
+                               flag ? (const int*)e->r_flag : nullptr, e->h_out_d);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (records) std::memcpy(records, e->h_out, (size_t)n * sizeof(double));
+        if (flag) *flag = e->h_out[n] != 0.0 ? 1 : 0;
+        return LJ_OK;
+    }
+    if (flag) HIP_TRY(hipMemcpyAsync(flag, e->r_flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(records, rec, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return LJ_OK;
+}
+
 // One velocity-Verlet step.  Its record is written during the next step's
 // cell build (or before the next step's integration without cells) or by
 // enqueue_finalize(e, 0), which must follow the last step of a sequence
@@ -1033,6 +1068,9 @@ int lj_engine_create(const lj_system_desc* systems, int nsys, int precision, lj_
         return cleanup_fail(fail(LJ_E_HIP, "hipStreamCreate failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return cleanup_fail(fail(LJ_E_HIP, "hipEventCreate failed"));
+    if (hipHostMalloc((void**)&e->h_out, (kHostOut + 1) * sizeof(double), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&e->h_out_d, e->h_out, 0) != hipSuccess)
+        return cleanup_fail(fail(LJ_E_HIP, "pinned record output: hipHostMalloc failed"));
     const size_t ns = e->nslot;
     if ((rc = dev_alloc(&e->d_sys, nsys)) || (rc = dev_alloc(&e->d_blk_sys, blk)) ||
         (rc = dev_alloc(&e->d_cblk_sys, cblk)) || (rc = dev_alloc(&e->d_sblk_sys, sblk)) ||
@@ -1143,6 +1181,7 @@ int lj_engine_destroy(lj_engine* e)
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->h_out) (void)hipHostFree(e->h_out);
     delete e;
     return LJ_OK;
 }
@@ -1628,11 +1667,7 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     e->res_par = r;
     HIP_TRY(hipGetLastError());
     int flag = 0;
-    HIP_TRY(hipMemcpyAsync(&flag, e->r_flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    if (records)
-        HIP_TRY(hipMemcpyAsync(records, f.records, (size_t)nsteps * nsys * sizeof(lj_thermo),
-                               hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    if ((rc = finish_call(e, nsteps, records, &flag))) return rc;
     if (!flag) {
         e->res_since += nsteps;
         return LJ_OK;
@@ -1713,11 +1748,7 @@ int run_steps_packed(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     }
     if ((rc = enqueue_finalize(e, 0))) return rc;   // the last step's record
     HIP_TRY(hipGetLastError());
-    if (records)
-        HIP_TRY(hipMemcpyAsync(records, f.records, (size_t)nsteps * e->nsys * sizeof(lj_thermo),
-                               hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    return LJ_OK;
+    return finish_call(e, nsteps, records, nullptr);
 }
 
 }  // namespace

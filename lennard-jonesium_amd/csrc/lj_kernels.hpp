@@ -2419,6 +2419,19 @@ __global__ __launch_bounds__(kBlock) void k_begin_call(double* v, long long n, i
     for (long long k = t0; k < nsys; k += stride) step[k] = 0;
 }
 
+// The end of a step call: its n record doubles and, when given, the resident
+// overflow flag (as the double after them) into `out`, pinned host memory
+// written through the device's mapping; one launch in place of two copies to
+// the host.
+__global__ __launch_bounds__(kBlock) void k_records_out(const double* rec, long long n,
+                                                        const int* flag, double* out)
+{
+    const long long t0 = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long k = t0; k < n; k += stride) out[k] = rec[k];
+    if (t0 == 0) out[n] = flag ? (double)*flag : 0.0;
+}
+
 // Copies every system's time, PE and W (n each) to ck: the clocks of a
 // resident snapshot, put back if a later call must be redone.
 __global__ __launch_bounds__(kBlock) void k_res_checkpoint(FinalArgs f, double* ck)
