@@ -1610,9 +1610,11 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
         // the first half kick and drift from the stored forces, then the fix-up
         r = e->res_par;
         const ljk::StepArgs p = e->args_res(r);
+        constexpr int cpb = ljk::kResDriftCells * (kBlock / 64);   // cells per block
+        const dim3 drift_grid((e->ncell_total + cpb - 1) / cpb);
         if ((rc = timed(e, 12, "res_drift", [&] {
-                 if (mixed) launch(e, ljk::k_res_kick_drift<true>, cell_grid, dim3(kBlock), p);
-                 else launch(e, ljk::k_res_kick_drift<false>, cell_grid, dim3(kBlock), p);
+                 if (mixed) launch(e, ljk::k_res_kick_drift<true>, drift_grid, dim3(kBlock), p);
+                 else launch(e, ljk::k_res_kick_drift<false>, drift_grid, dim3(kBlock), p);
              })))
             return rc;
         const ljk::StepArgs q = e->args_res(r, true);

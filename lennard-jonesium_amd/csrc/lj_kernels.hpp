@@ -1046,8 +1046,9 @@ constexpr int kPairBatch = 4;
 constexpr int kPairBatch64 = 2;
 
 // Visits every in-cutoff neighbour j (with its image) of sorted slot s in
-// double precision, evaluating r^2 exactly as the reference's pair filter:
-// sep = (r_i - r_j) - image * L, r^2 = (x^2 + y^2) + z^2, keep if r^2 < rc^2.
+// double precision, with the reference pair filter's formula in the oracle's
+// fixed order (engine/particle_pair_filter.cpp:172-187): sep = (r_i - r_j) -
+// image * L, r^2 = (x^2 + y^2) + z^2, keep if r^2 < rc^2.
 template <class Visit>
 __device__ __forceinline__ void for_each_neighbour_fp64(const SysDev& S, const StepArgs& p,
                                                         int s, Visit&& visit)
@@ -1253,26 +1254,31 @@ __device__ __forceinline__ void res_place(const StepArgs& p, const SysDev& S, in
 }
 
 // The first half kick and drift of a call that starts on live resident cells
-// (the previous call's last step stored its forces in the resident slots):
-// one wave per cell, lane t for slot rc_base[c] + t, with the arithmetic of
-// the key pass and the sort (advance), then res_place.  The fix-up follows.
+// (the previous call's last step stored its forces in the resident slots),
+// with the arithmetic of the key pass and the sort (advance), then res_place.
+// kResDriftCells cells per wave, 16 lanes each (lane t: slots t, t + 16, ...
+// of its cell).  The fix-up follows.
+constexpr int kResDriftCells = 4;
+
 template <bool MIXED_>
 __global__ __launch_bounds__(kBlock) void k_res_kick_drift(StepArgs p)
 {
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = (blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kResDriftCells + (lane >> 4);
     if (c >= p.ncell_total) return;
     const SysDev& S = p.sys[p.cell_sys[c]];
     if (c - S.cell_off >= S.ncell) return;   // tail cell: padding slots
-    if (lane >= p.rc_count[c]) return;
-    const int si = p.rc_base[c] + lane;
-    double x = p.pos.x[si], y = p.pos.y[si], z = p.pos.z[si];
-    double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
-    const double fx = p.force.x[si], fy = p.force.y[si], fz = p.force.z[si];
-    const double dx = p.disp.x[si], dy = p.disp.y[si], dz = p.disp.z[si];
-    double ix, iy, iz;
-    advance<true>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
-    res_place<MIXED_>(p, S, c, si, x, y, z, vx, vy, vz, dx + ix, dy + iy, dz + iz);
+    const int n = p.rc_count[c], base = p.rc_base[c];
+    for (int t = lane & 15; t < n; t += 16) {
+        const int si = base + t;
+        double x = p.pos.x[si], y = p.pos.y[si], z = p.pos.z[si];
+        double vx = p.vel.x[si], vy = p.vel.y[si], vz = p.vel.z[si];
+        const double fx = p.force.x[si], fy = p.force.y[si], fz = p.force.z[si];
+        const double dx = p.disp.x[si], dy = p.disp.y[si], dz = p.disp.z[si];
+        double ix, iy, iz;
+        advance<true>(S, x, y, z, vx, vy, vz, fx, fy, fz, ix, iy, iz);
+        res_place<MIXED_>(p, S, c, si, x, y, z, vx, vy, vz, dx + ix, dy + iy, dz + iz);
+    }
 }
 
 
