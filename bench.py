@@ -162,8 +162,8 @@ def energy_gate(eng, specs, states, precision, pair_kernel="tiled"):
 
 def step_kernel_ms(kt):
     """Kernel time per step (ms): the kernels launched once per step (a
-    resident call's cell build and packing, and the final record sums, run
-    once per call and are left out)."""
+    resident call's kick-drift and the final record sums run once per call
+    and are left out)."""
     return kt["per_step_ms"]
 
 
@@ -308,7 +308,7 @@ def liquid(args, eng):
             "frac": achieved / PEAK["fp32"], "traffic": traffic,
             "peak_unpacked_fp32": PEAK["fp32_unpacked"],
             "frac_of_unpacked": achieved / PEAK["fp32_unpacked"],
-            "kernel": (("k_pair_cells_mixed<LJ,2,4,1,RES> (pair forces, kicks, drift of "
+            "kernel": (("k_pair_cells_mixed<LJ,2,4,1,RES,UNIT> (pair forces, kicks, drift of "
                         "resident cells)" if resident else "k_pair_cells_mixed<LJ,KICK>")
                        if args.pair_kernel == "tiled" else "k_pair<MIXED,LJ,KICK>"),
             "kernel_ms": pair_ms, "flops_per_launch": flops,
@@ -318,7 +318,7 @@ def liquid(args, eng):
     if res64:
         a64 = flops / (kt64["mean_ms"]["pair"] * 1e-3) / 1e12
         res64["roofline"] = {"bound": "valu",
-                             "kernel": ("k_pair_cells_fp64<LJ,2,1,RES>" if resident
+                             "kernel": ("k_pair_cells_fp64<LJ,2,1,RES,UNIT>" if resident
                                         else "k_pair_cells_fp64<LJ,KICK>"),
                              "achieved": a64, "peak": PEAK["fp64"], "unit": "TFLOP/s",
                              "frac": a64 / PEAK["fp64"], "kernel_ms": kt64["mean_ms"]["pair"],

@@ -230,12 +230,14 @@ int lj_engine_set_graphs(lj_engine* e, int enabled);
  * all use the tiled pair kernel (all LJ_PAIR_CELL_TILED or all
  * LJ_PAIR_CELL_MULTIWAVE) builds its cell lists once, keeps every cell in a
  * fixed block of slots and, in each later step, moves only the particles that
- * changed cell (in place of the key pass, scan and sort); the call's last step
- * packs the state back.  Records and state are the same bit for bit with any
- * mode.  A call in which a cell would exceed its slots (48 members) or receive
- * more than 8 particles in one step is redone from its starting state with
- * the full cell build (same results, more time).  The first such call
- * allocates the resident buffers (about 150 bytes per cell slot, 48 slots per
+ * changed cell (in place of the key pass, scan and sort).  The cells stay live
+ * for later step calls of any length; every other entry point that reads or
+ * changes the state packs them back first.  Records and state are the same
+ * bit for bit with any mode.  When a cell would exceed its slots (48 members)
+ * or receive more than 8 particles in one step, the steps taken since the
+ * cells were entered are redone with the full cell build (same results, more
+ * time; records already returned are not touched).  The first resident call
+ * allocates the resident buffers (about 175 bytes per cell slot, 48 slots per
  * cell).  Modes: LJ_RESIDENT_OFF; LJ_RESIDENT_AUTO (default): when the engine
  * holds at least 10 particles per cell (liquid densities; dilute systems gain
  * nothing); LJ_RESIDENT_ON: whenever the kernels allow it.  lj_engine_autotune
@@ -249,8 +251,8 @@ int lj_engine_set_resident_cells(lj_engine* e, int mode);
 /* The resident-cell mode (LJ_RESIDENT_OFF for a null engine). */
 int lj_engine_resident_cells(const lj_engine* e);
 
-/* Calls run on resident cells since creation, and how many of them were
- * redone with the full cell build. */
+/* Step calls run on resident cells since creation, and how many of them
+ * were redone with the full cell build. */
 int lj_engine_resident_stats(const lj_engine* e, long long* calls, long long* fallbacks);
 
 /* Pair-kernel variant: LJ_PAIR_CELL_TILED (default) runs one wave64 per home
@@ -308,8 +310,8 @@ int lj_engine_set_profiling(lj_engine* e, int enabled);
 int lj_engine_kernel_times(lj_engine* e, int cap, const char** names, double* ms, int* count);
 
 /* Launch count of each kernel of lj_engine_kernel_times, in the same order
- * (per-step kernels launch once per step, the cell build of a resident call
- * once per call). */
+ * (per-step kernels launch once per step; a resident call's kick-drift and
+ * record finish once per call). */
 int lj_engine_kernel_launches(lj_engine* e, int cap, long long* launches, int* count);
 
 #ifdef __cplusplus

@@ -223,10 +223,11 @@ class Engine:
         check(self.lib.lj_engine_set_graphs(self.h, 1 if enabled else 0))
 
     def set_resident_cells(self, mode):
-        """Resident cells for step calls of 4 or more steps: the cell lists are
-        built once per call and only the particles that change cell are moved
-        afterwards; results are the same bit for bit.  mode: False / 'off',
-        'auto' (default: at 10 or more particles per cell), True / 'on'."""
+        """Resident cells, entered by a step call of 4 or more steps: the cell
+        lists are built once and afterwards, across later step calls too, only
+        the particles that change cell are moved; results are the same bit for
+        bit.  mode: False / 'off', 'auto' (default: at 10 or more particles
+        per cell), True / 'on'."""
         modes = {False: 0, "off": 0, "auto": 1, True: 2, "on": 2}
         check(self.lib.lj_engine_set_resident_cells(self.h, modes[mode]))
 

@@ -190,7 +190,7 @@ def test_resident_coexistence_2048k(precision):
 
 
 def test_resident_fallback_when_cells_overflow():
-    """A cutoff of 5 sigma at rho 0.8442 puts about 105 particles in a cell,
+    """A cutoff of 5 sigma at rho 0.8442 puts about 148 particles in a cell,
     beyond the 48 resident slots: the call is redone with the full build and
     its results equal a run with resident cells off."""
     eng = _eng()
