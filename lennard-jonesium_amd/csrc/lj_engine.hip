@@ -1335,6 +1335,10 @@ constexpr int kResMinSteps = 4;
 // coexistence workload (8.2 per cell, half of the cells nearly empty: 521 ->
 // 599 us, the fix-up and the per-cell drift outweighing the saved sort).
 constexpr double kResMinOccupancy = 10.0;
+// Steps on live resident cells after which the next call packs them back and
+// enters again (the same results), so that a capacity overflow never redoes
+// more than about this many steps.
+constexpr long long kResMaxSince = 16384;
 
 bool res_eligible(const lj_engine* e, int nsteps)
 {
@@ -1706,6 +1710,8 @@ int run_steps_packed(lj_engine* e, int nsteps, bool graphs, lj_thermo* records);
 int run_steps(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
 {
     int rc;
+    // a fresh snapshot now and then bounds what an overflow has to redo
+    if (e->res_live && e->res_since >= kResMaxSince && (rc = res_settle(e))) return rc;
     if (e->res_live || (res_eligible(e, nsteps) && (e->res_ready || !e->rec_buf))) {
         if ((rc = res_ensure(e))) return rc;
         const long long since = e->res_live ? e->res_since : 0;

@@ -104,9 +104,10 @@ def test_resident_split_calls_and_eager(precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_resident_live_between_calls(precision):
     """Between resident calls: reading the state, measuring, a velocity
-    rescale and a call of 1 step.  Each reads or changes the packed state
-    (the live cells are packed back first) and the steps after it continue
-    exactly as on the full-build path."""
+    rescale, a call of 1 step, a g(r) sample, the cell lists and the
+    neighbour counts.  Each reads or changes the packed state (the live cells
+    are packed back first) and the steps after it continue exactly as on the
+    full-build path."""
     eng = _eng()
     systems = _systems(eng, [(32000, 0.8442, 0.72, 5489, 2.5), (4000, 0.8, 1.0, 11, 2.5)])
     _say("resident live between calls,", precision)
@@ -124,13 +125,19 @@ def test_resident_live_between_calls(precision):
         got.append(e.step(4))
         e.set_temperature(0, 0.9)
         got.append(e.step(12))
+        e.rdf_sample(4.0, 200)
+        got.append(e.rdf_read(200)[0])
+        got.append(e.step(6))
+        got.extend(e.cell_list(1))
+        got.append(e.step(5))
+        got.append(e.neighbor_counts(0))
         got.append(e.get_state(0))
         got.append(e.get_state(1))
         got.append(e.resident_stats())
         e.close()
         out.append(got)
     a, b = out
-    assert a[-1] == (4, 0) and b[-1] == (0, 0), (a[-1], b[-1])
+    assert a[-1] == (6, 0) and b[-1] == (0, 0), (a[-1], b[-1])
     for x, y in zip(a[:-1], b[:-1]):
         if isinstance(x, dict):
             for f in FIELDS:
@@ -138,6 +145,19 @@ def test_resident_live_between_calls(precision):
             assert (x["time"], x["pe"], x["vir"]) == (y["time"], y["pe"], y["vir"])
         else:
             assert np.array_equal(x, y)
+
+
+def test_resident_snapshot_refresh():
+    """After 16,384 steps on live cells the next call packs them back and
+    enters again: calls of 9,000, 9,000 and 100 steps (the third re-enters)
+    equal the full-build run."""
+    eng = _eng()
+    systems = _systems(eng, [(4000, 0.8, 1.0, 11, 2.5)])
+    _say("resident snapshot refresh")
+    a = _run(eng, systems, "mixed", [9000, 9000, 100], True)
+    b = _run(eng, systems, "mixed", [9000, 9000, 100], False)
+    assert a[2] == (3, 0), f"resident calls / fallbacks: {a[2]}"
+    _same(a, b)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
