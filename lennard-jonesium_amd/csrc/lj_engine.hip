@@ -1518,12 +1518,10 @@ int res_settle(lj_engine* e)
 {
     if (!e->res_live) return LJ_OK;
     int rc;
-    const int nsys = e->nsys;
     ljk::StepArgs p = e->args(e->cur ^ 1);
     p.count = e->r_count;
     p.count_init = e->r_count;   // the two-level scan's count reset leaves them as they are
-    const ljk::FinalArgs f = e->fargs(0);
-    (void)nsys;
+    const ljk::FinalArgs f = e->fargs(0);   // (no record blocks in these launches)
     if (!e->two_level_scan()) {
         if ((rc = timed(e, 1, "scan", [&] {
                  launch(e, ljk::k_scan_one, dim3(1), dim3(ljk::kScanOneThreads), p, f);
