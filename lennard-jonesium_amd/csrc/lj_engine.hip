@@ -1577,12 +1577,12 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
     const ljk::FinalArgs f = e->fargs(0);
     const long long nrec = (long long)nsteps * nsys * 7;
     const long long nthr = std::max<long long>(nrec, nsys);
-    hipLaunchKernelGGL(ljk::k_begin_call,
-                       dim3((unsigned)std::min<long long>(4096, (nthr + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, e->stream, f.records, nrec, e->d_step, nsys);
     const dim3 cell_grid((e->ncell_total + kBlock / 64 - 1) / (kBlock / 64));
     int r;
     if (!e->res_live) {
+        hipLaunchKernelGGL(ljk::k_begin_call,
+                           dim3((unsigned)std::min<long long>(4096, (nthr + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, e->stream, f.records, nrec, e->d_step, nsys);
         // the snapshot: the packed set e->cur and d_force stay as they are;
         // the clocks are checkpointed
         hipLaunchKernelGGL(ljk::k_res_checkpoint, dim3((nsys + kBlock - 1) / kBlock), dim3(kBlock),
@@ -1614,9 +1614,14 @@ int run_steps_res(lj_engine* e, int nsteps, bool graphs, lj_thermo* records)
         const ljk::StepArgs p = e->args_res(r);
         constexpr int cpb = ljk::kResDriftCells * (kBlock / 64);   // cells per block
         const dim3 drift_grid((e->ncell_total + cpb - 1) / cpb);
+        // (this launch also clears the call's records: k_begin_call's work)
         if ((rc = timed(e, 12, "res_drift", [&] {
-                 if (mixed) launch(e, ljk::k_res_kick_drift<true>, drift_grid, dim3(kBlock), p);
-                 else launch(e, ljk::k_res_kick_drift<false>, drift_grid, dim3(kBlock), p);
+                 if (mixed)
+                     launch(e, ljk::k_res_kick_drift<true>, drift_grid, dim3(kBlock), p, f.records,
+                            nrec, e->d_step, nsys);
+                 else
+                     launch(e, ljk::k_res_kick_drift<false>, drift_grid, dim3(kBlock), p, f.records,
+                            nrec, e->d_step, nsys);
              })))
             return rc;
         const ljk::StepArgs q = e->args_res(r, true);

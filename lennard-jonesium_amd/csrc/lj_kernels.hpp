@@ -1257,12 +1257,21 @@ __device__ __forceinline__ void res_place(const StepArgs& p, const SysDev& S, in
 // (the previous call's last step stored its forces in the resident slots),
 // with the arithmetic of the key pass and the sort (advance), then res_place.
 // kResDriftCells cells per wave, 16 lanes each (lane t: slots t, t + 16, ...
-// of its cell).  The fix-up follows.
+// of its cell).  The fix-up follows.  As the call's first kernel it also does
+// k_begin_call's work: clears the call's nrec record doubles and resets every
+// system's record counter.
 constexpr int kResDriftCells = 4;
 
 template <bool MIXED_>
-__global__ __launch_bounds__(kBlock) void k_res_kick_drift(StepArgs p)
+__global__ __launch_bounds__(kBlock) void k_res_kick_drift(StepArgs p, double* rec, long long nrec,
+                                                           int* step, int nsys)
 {
+    {
+        const long long t0 = (long long)blockIdx.x * kBlock + threadIdx.x;
+        const long long stride = (long long)gridDim.x * kBlock;
+        for (long long k = t0; k < nrec; k += stride) rec[k] = 0.0;
+        for (long long k = t0; k < nsys; k += stride) step[k] = 0;
+    }
     const int lane = threadIdx.x & 63;
     const int c = (blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kResDriftCells + (lane >> 4);
     if (c >= p.ncell_total) return;
