@@ -102,6 +102,20 @@ def test_resident_split_calls_and_eager(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resident_split_calls_multiwave(precision):
+    """The four-wave small-system kernels on live cells (N = 864 and 4,000 in
+    one engine): calls of 5, 3, 17, 1 and 40 steps equal one call of 66 steps
+    with the full build."""
+    eng = _eng()
+    systems = _systems(eng, [(864, 0.8, 1.0, 5489, 2.5), (4000, 0.8, 1.0, 17, 2.5)])
+    _say("resident split calls multiwave,", precision)
+    ref = _run(eng, systems, precision, [66], False, kernel="multiwave")
+    a = _run(eng, systems, precision, [5, 3, 17, 1, 40], True, kernel="multiwave")
+    assert a[2] == (5, 0), f"resident calls / fallbacks: {a[2]}"
+    _same(a, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_resident_live_between_calls(precision):
     """Between resident calls: reading the state, measuring, a velocity
     rescale, a call of 1 step, a g(r) sample, the cell lists and the
